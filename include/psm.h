@@ -1,0 +1,147 @@
+/*
+ * psm.h -- C ABI of the MI355X (gfx950) Kirchhoff post-stack 3D time
+ * migration library, libpsm_hip.so.
+ *
+ * Plain C: pointers, sizes and status codes; no HIP or C++ types.  Every call
+ * returns PSM_OK (0) or a nonzero PSM_ERR_* code; psm_last_error(ctx) then
+ * holds a message.  A context owns one HIP stream and the device buffers its
+ * calls need; calls on one context are serialised by an internal lock, calls
+ * on different contexts may run concurrently (one context per host thread is
+ * the intended use from OpenMP loops such as the reference's main.cpp:255).
+ *
+ * Reference interfaces replaced (sergeevsn/poststackmig3d):
+ *   psm_kirchhoff_inline     MigrationKernel::kirchhoffKernel3DWithPadding,
+ *                            include/migration_kernel.h:16-31,
+ *                            src/migration_kernel.cpp:17-383
+ *   psm_migrate_volume       the output-inline loop of src/main.cpp:254-383
+ *                            together with the buffer staging it drives,
+ *                            MigrationKernel::prepareInputBufferWithPadding
+ *                            (migration_kernel.cpp:385-468) and
+ *                            prepareVelocitySliceWithPadding (:470-511), and
+ *                            the geometry of main.cpp:140-217
+ *                            (output labels, estimateMaxVelocity :22-64,
+ *                            inline aperture window :188-202, t0 table
+ *                            :214-217)
+ *   psm_survey_plan          main.cpp:140-217 alone (geometry only)
+ */
+#ifndef PSM_H
+#define PSM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSM_OK 0
+#define PSM_ERR_ARG 1     /* invalid argument (null pointer, bad size, ...) */
+#define PSM_ERR_HIP 2     /* a HIP runtime call failed */
+#define PSM_ERR_NOMEM 3   /* device or host allocation failed */
+#define PSM_ERR_NODEV 4   /* no usable gfx950 device */
+
+#define PSM_HOST 0        /* pointer arguments are host memory */
+#define PSM_DEVICE 1      /* pointer arguments are device memory (HBM) */
+
+typedef struct psm_ctx psm_ctx;
+
+/* Library / context lifetime. */
+const char* psm_version(void);
+int psm_create(psm_ctx** ctx, int device);
+void psm_destroy(psm_ctx* ctx);
+const char* psm_last_error(const psm_ctx* ctx);
+/* Use an existing HIP stream (hipStream_t passed as void*, NULL = the
+ * context's own stream) for every later call on this context. */
+int psm_set_stream(psm_ctx* ctx, void* stream);
+int psm_synchronize(psm_ctx* ctx);
+
+/*
+ * Per-inline drop-in for MigrationKernel::kirchhoffKernel3DWithPadding
+ * (migration_kernel.h:16-31).  Same arguments in the same order, with the
+ * std::vector arguments passed as pointer + length and the output
+ * vector<vector<float>> [n_xl][n_t] passed as one contiguous host array.
+ * xl_pos_indices is unused by the reference (migration_kernel.cpp:23) and is
+ * accepted for signature parity (may be NULL).  All pointers are host
+ * memory; the call is synchronous and fully overwrites output_slice.
+ * Semantics are the reference's OpenMP build as main.cpp drives it: the
+ * crosslines of one call are walked in order with per-sample parameter
+ * caches that are not reset between crosslines (migration_kernel.cpp:64-152).
+ */
+int psm_kirchhoff_inline(psm_ctx* ctx,
+                         const float* input_buffer,
+                         size_t n_buf_il, size_t n_buf_xl, size_t n_t,
+                         const float* velocity_slice,
+                         float* output_slice, size_t n_xl,
+                         const int32_t* il_pos_indices, size_t n_il_pos,
+                         const int32_t* xl_pos_indices, size_t n_xl_pos,
+                         int32_t current_il_pos, int32_t current_xl_start,
+                         float dt, const float* t0_times,
+                         float tan_theta_sq, float inline_bin_size,
+                         float crossline_bin_size, int amp_correction);
+
+/*
+ * Survey description for the whole-volume path: the input grid (real traces
+ * only, [n_il][n_xl][n_t], sorted labels) and the Config fields of
+ * config_parser.h:7-20 that reach the kernel.
+ */
+typedef struct psm_survey {
+    size_t n_il, n_xl, n_t;
+    double dt;                       /* seconds (LookupTableResult::dt_seconds) */
+    double inline_step;              /* metres */
+    double crossline_step;           /* metres */
+    int inline_padding;              /* traces per side */
+    int crossline_padding;           /* traces per side */
+    double angle_aperture;           /* degrees */
+    int amp_correction;              /* 0/1 */
+    const int32_t* inline_labels;    /* host [n_il], sorted; NULL = 1..n_il */
+    const int32_t* crossline_labels; /* host [n_xl], sorted; NULL = 1..n_xl */
+    float max_velocity;              /* estimateMaxVelocity (main.cpp:22-64);
+                                        <= 0: probe the velocity cube */
+} psm_survey;
+
+/* Geometry derived from a survey (main.cpp:140-217).  Arrays are owned by the
+ * plan and freed by psm_plan_free. */
+typedef struct psm_plan {
+    size_t n_out_il, n_out_xl, n_t;
+    int max_aperture_il;             /* main.cpp:201 */
+    float tan_theta_sq;              /* main.cpp:188-189 */
+    float max_velocity;              /* main.cpp:192-195 */
+    int32_t* out_inline_labels;      /* [n_out_il] */
+    int32_t* out_crossline_labels;   /* [n_out_xl] */
+    int32_t* out_to_in_il;           /* [n_out_il] real inline used (main.cpp:258-288) */
+    int32_t* window;                 /* [2*n_out_il] data inline window [b0,b1) */
+    float* t0_times;                 /* [n_t] */
+} psm_plan;
+
+/* vel_probe: the velocity cube (host or device per `where`) used when
+ * survey->max_velocity <= 0; may be NULL otherwise. */
+int psm_survey_plan(const psm_survey* survey, const float* vel_probe, int where,
+                    psm_plan* plan);
+void psm_plan_free(psm_plan* plan);
+
+/*
+ * Whole-volume migration.  data and vel are [n_il][n_xl][n_t] float32 cubes
+ * on the input grid (vel = Vrms of each image point, as the reference's
+ * velocity provider hands it over per inline); out is
+ * [n_out_il][n_out_xl][n_t] (psm_survey_plan gives the shape).  `where`
+ * says whether the three pointers are host or device memory.  With
+ * PSM_DEVICE the call enqueues on the context's stream and returns once the
+ * work is queued; with PSM_HOST it copies, computes, copies back and
+ * returns when out is valid.  Results equal main.cpp's loop over
+ * kirchhoffKernel3DWithPadding on the same cube.
+ */
+int psm_migrate_volume(psm_ctx* ctx, const psm_survey* survey,
+                       const float* data, const float* vel, float* out,
+                       int where);
+
+/* Pair counts of the work unit for a survey: p_ap = (output sample, real
+ * aperture trace) pairs inside the aperture mask, p_g = those that also
+ * gather (i < n_t - 1).  vel is host or device per `where`. */
+int psm_count_pairs(psm_ctx* ctx, const psm_survey* survey, const float* vel,
+                    int where, double* p_ap, double* p_g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSM_H */

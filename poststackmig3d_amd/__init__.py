@@ -1,0 +1,19 @@
+"""poststackmig3d_amd -- MI355X (gfx950) Kirchhoff post-stack 3D time migration.
+
+The product is libpsm_hip.so (C ABI in include/psm.h) plus the poststackmig
+driver; this package holds their sources (csrc/) and a thin Python host
+mirror of the reference's migration-kernel interface (kernel.py).
+"""
+from ._lib import PsmError, context, load  # noqa: F401
+from .kernel import (  # noqa: F401
+    SurveyParams,
+    count_pairs,
+    kirchhoff_kernel_3d_with_padding,
+    migrate_volume,
+    survey_plan,
+)
+
+__all__ = [
+    "PsmError", "context", "load", "SurveyParams", "count_pairs",
+    "kirchhoff_kernel_3d_with_padding", "migrate_volume", "survey_plan",
+]

@@ -1,0 +1,138 @@
+"""ctypes binding of libpsm_hip.so (the C ABI declared in include/psm.h).
+
+The library is built in-tree by __graft_entry__.build() (make -C
+poststackmig3d_amd/csrc).  There is no CPU fallback: if the library is missing
+or no gfx950 device is usable, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsm_hip.so")
+
+PSM_OK = 0
+PSM_HOST = 0
+PSM_DEVICE = 1
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+
+
+class Survey(C.Structure):
+    """psm_survey (include/psm.h)."""
+    _fields_ = [
+        ("n_il", C.c_size_t), ("n_xl", C.c_size_t), ("n_t", C.c_size_t),
+        ("dt", C.c_double), ("inline_step", C.c_double), ("crossline_step", C.c_double),
+        ("inline_padding", C.c_int), ("crossline_padding", C.c_int),
+        ("angle_aperture", C.c_double), ("amp_correction", C.c_int),
+        ("inline_labels", _ip), ("crossline_labels", _ip),
+        ("max_velocity", C.c_float),
+    ]
+
+
+class Plan(C.Structure):
+    """psm_plan (include/psm.h)."""
+    _fields_ = [
+        ("n_out_il", C.c_size_t), ("n_out_xl", C.c_size_t), ("n_t", C.c_size_t),
+        ("max_aperture_il", C.c_int), ("tan_theta_sq", C.c_float), ("max_velocity", C.c_float),
+        ("out_inline_labels", _ip), ("out_crossline_labels", _ip), ("out_to_in_il", _ip),
+        ("window", _ip), ("t0_times", _fp),
+    ]
+
+
+class PsmError(RuntimeError):
+    pass
+
+
+_lib = None
+_lock = threading.Lock()
+
+# (name, restype, argtypes) of every exported symbol
+SIGNATURES = [
+    ("psm_version", C.c_char_p, []),
+    ("psm_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    ("psm_destroy", None, [C.c_void_p]),
+    ("psm_last_error", C.c_char_p, [C.c_void_p]),
+    ("psm_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("psm_synchronize", C.c_int, [C.c_void_p]),
+    ("psm_kirchhoff_inline", C.c_int, [
+        C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p,
+        C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int32, C.c_int32,
+        C.c_float, C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_int]),
+    ("psm_survey_plan", C.c_int, [C.POINTER(Survey), C.c_void_p, C.c_int, C.POINTER(Plan)]),
+    ("psm_plan_free", None, [C.POINTER(Plan)]),
+    ("psm_migrate_volume", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_int]),
+    ("psm_count_pairs", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p, C.c_int,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+]
+
+
+def load():
+    """Loads libpsm_hip.so and declares its signatures (no device is touched)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise PsmError(
+                    f"libpsm_hip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+            L = C.CDLL(LIB_PATH)
+            for name, res, args in SIGNATURES:
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+class Context:
+    """One psm_ctx (a HIP stream plus device buffers) on one device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.psm_create(C.byref(h), int(device))
+        if rc != PSM_OK:
+            raise PsmError(f"psm_create(device={device}) failed with code {rc} "
+                           "(no usable gfx950 device?)")
+        self.h = h
+        self.device = device
+
+    def check(self, rc, what):
+        if rc != PSM_OK:
+            msg = self.lib.psm_last_error(self.h)
+            raise PsmError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def set_stream(self, stream_ptr: int | None):
+        self.check(self.lib.psm_set_stream(self.h, C.c_void_p(stream_ptr or 0)), "psm_set_stream")
+
+    def synchronize(self):
+        self.check(self.lib.psm_synchronize(self.h), "psm_synchronize")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.psm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_ctx = {}
+
+
+def context(device: int = 0) -> Context:
+    """Process-wide default context per device."""
+    with _lock:
+        c = _ctx.get(device)
+    if c is None:
+        c = Context(device)
+        with _lock:
+            _ctx[device] = c
+    return c

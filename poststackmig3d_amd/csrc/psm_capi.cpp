@@ -1,0 +1,499 @@
+// psm_capi.cpp -- the C ABI of libpsm_hip.so (declared in include/psm.h).
+//
+// Host side of the drop-in boundary: argument checks, the geometry of the
+// reference driver (main.cpp:140-217, 258-288; migration_kernel.cpp:404-409),
+// device buffer management and kernel dispatch.  No CPU fallback: every
+// migration runs on the gfx950 kernels of psm_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "psm.h"
+#include "psm_internal.h"
+
+namespace {
+
+// Grow-only device allocation owned by a context.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct psm_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::string err;
+  DevBuf raw, pairs, vel, out, t0, tasks, rowpos, counters, zero;
+  size_t zero_nt = 0;
+};
+
+namespace {
+
+int fail(psm_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define PSM_HIP(ctx, call)                                                           \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail((ctx), PSM_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define PSM_ALLOC(ctx, buf, bytes)                                                        \
+  do {                                                                                    \
+    hipError_t e_ = (buf).ensure(bytes);                                                  \
+    if (e_ != hipSuccess)                                                                 \
+      return fail((ctx), PSM_ERR_NOMEM, "device allocation of %zu bytes failed: %s",      \
+                  (size_t)(bytes), hipGetErrorString(e_));                                \
+  } while (0)
+
+// ---------------------------------------------------------------- geometry
+// main.cpp:144-172: labels extended by `pad` steps of the first-two-label step.
+std::vector<int32_t> output_labels(const std::vector<int32_t>& labels, int pad) {
+  if (pad > 0 && labels.size() > 1) {
+    const int32_t step = labels[1] - labels[0];
+    std::vector<int32_t> out;
+    if (step <= 0) return out;
+    const int32_t first = labels[0] - pad * step;
+    const int32_t last = labels.back() + pad * step;
+    for (int32_t l = first; l <= last; l += step) out.push_back(l);
+    return out;
+  }
+  return labels;
+}
+
+// main.cpp:258-288: index of the real inline an output inline uses.
+int32_t map_output_inline(const std::vector<int32_t>& labels, int32_t out_label) {
+  for (size_t i = 0; i < labels.size(); ++i)
+    if (labels[i] == out_label) return (int32_t)i;
+  if (out_label < labels.front()) return 0;
+  if (out_label > labels.back()) return (int32_t)labels.size() - 1;
+  size_t best = 0;
+  int32_t md = std::abs(out_label - labels[0]);
+  for (size_t i = 1; i < labels.size(); ++i) {
+    const int32_t d = std::abs(out_label - labels[i]);
+    if (d < md) {
+      md = d;
+      best = i;
+    }
+  }
+  return (int32_t)best;
+}
+
+std::vector<int32_t> labels_or_default(const int32_t* l, size_t n) {
+  std::vector<int32_t> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = l ? l[i] : (int32_t)(i + 1);
+  return v;
+}
+
+int check_survey(psm_ctx* c, const psm_survey* s) {
+  if (!s) return fail(c, PSM_ERR_ARG, "survey is NULL");
+  if (s->n_il == 0 || s->n_xl == 0 || s->n_t == 0)
+    return fail(c, PSM_ERR_ARG, "empty survey (%zu x %zu x %zu)", s->n_il, s->n_xl, s->n_t);
+  if (s->n_t > (1u << 20) || s->n_il > (1u << 20) || s->n_xl > (1u << 20))
+    return fail(c, PSM_ERR_ARG, "survey dimension too large");
+  if (!(s->dt > 0)) return fail(c, PSM_ERR_ARG, "dt must be positive, got %g", s->dt);
+  if (!(s->inline_step > 0) || !(s->crossline_step > 0))
+    return fail(c, PSM_ERR_ARG, "inline_step and crossline_step must be positive");
+  if (s->inline_padding < 0 || s->crossline_padding < 0)
+    return fail(c, PSM_ERR_ARG, "padding must be non-negative");
+  if (!(s->angle_aperture > 0 && s->angle_aperture < 90))
+    return fail(c, PSM_ERR_ARG, "angle_aperture must be in (0, 90), got %g", s->angle_aperture);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int32_t* l = pass ? s->crossline_labels : s->inline_labels;
+    const size_t n = pass ? s->n_xl : s->n_il;
+    if (!l) continue;
+    for (size_t i = 1; i < n; ++i)
+      if (l[i] <= l[i - 1]) return fail(c, PSM_ERR_ARG, "labels must be strictly increasing");
+  }
+  return PSM_OK;
+}
+
+int plan_impl(psm_ctx* c, const psm_survey* s, const float* vel_probe, int where,
+              psm_plan* plan) {
+  int rc = check_survey(c, s);
+  if (rc) return rc;
+  if (!plan) return fail(c, PSM_ERR_ARG, "plan is NULL");
+  std::memset(plan, 0, sizeof *plan);
+  const std::vector<int32_t> ill = labels_or_default(s->inline_labels, s->n_il);
+  const std::vector<int32_t> xll = labels_or_default(s->crossline_labels, s->n_xl);
+  const std::vector<int32_t> oil = output_labels(ill, s->inline_padding);
+  const std::vector<int32_t> oxl = output_labels(xll, s->crossline_padding);
+  if (oil.empty() || oxl.empty()) return fail(c, PSM_ERR_ARG, "empty output geometry");
+
+  // estimateMaxVelocity (main.cpp:22-64): 9 probes at the last sample
+  float vmax = s->max_velocity;
+  if (!(vmax > 0)) {
+    if (!vel_probe) return fail(c, PSM_ERR_ARG, "max_velocity <= 0 and no velocity cube to probe");
+    std::vector<size_t> ils{0}, xls{0};
+    if (s->n_il > 1) { ils.push_back(s->n_il / 2); ils.push_back(s->n_il - 1); }
+    if (s->n_xl > 1) { xls.push_back(s->n_xl / 2); xls.push_back(s->n_xl - 1); }
+    vmax = 0.0f;
+    for (size_t a : ils)
+      for (size_t b : xls) {
+        const size_t off = (a * s->n_xl + b) * s->n_t + (s->n_t - 1);
+        float v = 0.0f;
+        if (where == PSM_DEVICE) {
+          hipError_t e = hipMemcpy(&v, vel_probe + off, sizeof(float), hipMemcpyDeviceToHost);
+          if (e != hipSuccess)
+            return fail(c, PSM_ERR_HIP, "velocity probe copy failed: %s", hipGetErrorString(e));
+        } else {
+          v = vel_probe[off];
+        }
+        if (v > vmax) vmax = v;
+      }
+    if (vmax < 100.0f) vmax = 3000.0f;
+  }
+  // main.cpp:188-202
+  const float tan_theta = (float)std::tan(s->angle_aperture * 3.14159265358979323846 / 180.0);
+  const float tan2 = tan_theta * tan_theta;
+  const double max_t = (double)(s->n_t - 1) * s->dt;
+  const float radius = (vmax * (float)max_t / 2.0f) * std::sqrt(tan2);
+  const int ap = (int)((double)radius / s->inline_step) + 1;
+
+  plan->n_out_il = oil.size();
+  plan->n_out_xl = oxl.size();
+  plan->n_t = s->n_t;
+  plan->max_aperture_il = ap;
+  plan->tan_theta_sq = tan2;
+  plan->max_velocity = vmax;
+  plan->out_inline_labels = (int32_t*)std::malloc(oil.size() * sizeof(int32_t));
+  plan->out_crossline_labels = (int32_t*)std::malloc(oxl.size() * sizeof(int32_t));
+  plan->out_to_in_il = (int32_t*)std::malloc(oil.size() * sizeof(int32_t));
+  plan->window = (int32_t*)std::malloc(2 * oil.size() * sizeof(int32_t));
+  plan->t0_times = (float*)std::malloc(s->n_t * sizeof(float));
+  if (!plan->out_inline_labels || !plan->out_crossline_labels || !plan->out_to_in_il ||
+      !plan->window || !plan->t0_times) {
+    psm_plan_free(plan);
+    return fail(c, PSM_ERR_NOMEM, "host allocation failed");
+  }
+  std::memcpy(plan->out_inline_labels, oil.data(), oil.size() * sizeof(int32_t));
+  std::memcpy(plan->out_crossline_labels, oxl.data(), oxl.size() * sizeof(int32_t));
+  const int n_il = (int)s->n_il;
+  for (size_t io = 0; io < oil.size(); ++io) {
+    const int cidx = map_output_inline(ill, oil[io]);
+    plan->out_to_in_il[io] = cidx;
+    // migration_kernel.cpp:404-409
+    const int s_req = std::max(0, cidx - ap);
+    const int e_req = std::min(n_il, cidx + ap + 1);
+    plan->window[2 * io] = std::max(0, s_req - s->inline_padding);
+    plan->window[2 * io + 1] = std::min(n_il, e_req + s->inline_padding);
+  }
+  for (size_t i = 0; i < s->n_t; ++i) plan->t0_times[i] = (float)((double)i * s->dt);  // main.cpp:216
+  return PSM_OK;
+}
+
+// A trace of n_t zero pairs, read by the kernel in place of absent traces.
+hipError_t ensure_zero_trace(psm_ctx* c, size_t n_t, hipStream_t st) {
+  if (c->zero_nt >= n_t && c->zero.p) return hipSuccess;
+  hipError_t e = c->zero.ensure(n_t * sizeof(float2));
+  if (e != hipSuccess) return e;
+  e = hipMemsetAsync(c->zero.p, 0, n_t * sizeof(float2), st);
+  if (e == hipSuccess) c->zero_nt = n_t;
+  return e;
+}
+
+hipError_t upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
+  hipError_t e = b.ensure(bytes);
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* psm_version(void) { return "poststackmig3d_amd 0.1 (gfx950)"; }
+
+int psm_create(psm_ctx** out, int device) {
+  if (!out) return PSM_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PSM_ERR_NODEV;
+  if (device < 0 || device >= n) return PSM_ERR_NODEV;
+  psm_ctx* c = new (std::nothrow) psm_ctx();
+  if (!c) return PSM_ERR_NOMEM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return PSM_ERR_HIP;
+  }
+  c->stream = c->own_stream;
+  *out = c;
+  return PSM_OK;
+}
+
+void psm_destroy(psm_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (DevBuf* b : {&c->raw, &c->pairs, &c->vel, &c->out, &c->t0, &c->tasks, &c->rowpos, &c->counters, &c->zero})
+    b->release();
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+const char* psm_last_error(const psm_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int psm_set_stream(psm_ctx* c, void* stream) {
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+  return PSM_OK;
+}
+
+int psm_synchronize(psm_ctx* c) {
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  PSM_HIP(c, hipSetDevice(c->device));
+  PSM_HIP(c, hipStreamSynchronize(c->stream));
+  return PSM_OK;
+}
+
+int psm_kirchhoff_inline(psm_ctx* c, const float* input_buffer, size_t n_buf_il,
+                         size_t n_buf_xl, size_t n_t, const float* velocity_slice,
+                         float* output_slice, size_t n_xl, const int32_t* il_pos,
+                         size_t n_il_pos, const int32_t* xl_pos, size_t n_xl_pos,
+                         int32_t current_il_pos, int32_t current_xl_start, float dt,
+                         const float* t0_times, float tan_theta_sq, float inline_bin_size,
+                         float crossline_bin_size, int amp_correction) {
+  (void)xl_pos;
+  (void)n_xl_pos;
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n_xl == 0 || n_t == 0) return PSM_OK;  // migration_kernel.cpp:35-37
+  if (!output_slice || !t0_times || !velocity_slice)
+    return fail(c, PSM_ERR_ARG, "null output, velocity or t0 pointer");
+  if (n_buf_il > 0 && (!input_buffer || !il_pos))
+    return fail(c, PSM_ERR_ARG, "null input buffer or il_pos_indices");
+  if (n_il_pos < n_buf_il)
+    return fail(c, PSM_ERR_ARG, "il_pos_indices has %zu entries, need %zu", n_il_pos, n_buf_il);
+  if (n_t > (1u << 24) || n_buf_xl > (1u << 24) || n_xl > (1u << 24) || n_buf_il > (1u << 24))
+    return fail(c, PSM_ERR_ARG, "dimension too large");
+  if (!(dt > 0) || !(crossline_bin_size > 0))
+    return fail(c, PSM_ERR_ARG, "dt and crossline_bin_size must be positive");
+  PSM_HIP(c, hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+
+  const int32_t cpos = (current_il_pos >= 0 && (size_t)current_il_pos < n_il_pos)
+                           ? il_pos[current_il_pos] : 0;  // migration_kernel.cpp:48-49
+  bool contiguous = n_buf_il > 0;
+  for (size_t b = 1; b < n_buf_il && contiguous; ++b)
+    contiguous = il_pos[b] == il_pos[0] + (int32_t)b;
+  long long crow = n_buf_il ? (long long)cpos - il_pos[0] : 0;
+  if (std::llabs(crow) > (1LL << 28)) contiguous = false;
+
+  const size_t n_traces = n_buf_il * n_buf_xl;
+  const size_t in_bytes = n_traces * n_t * sizeof(float);
+  PSM_ALLOC(c, c->pairs, std::max<size_t>(n_traces * n_t * sizeof(float2), 8));
+  if (n_traces) {
+    PSM_HIP(c, upload(c->raw, input_buffer, in_bytes, st));
+    PSM_HIP(c, psm::launch_pack_pairs(c->raw.as<float>(), c->pairs.as<float2>(), n_traces, (int)n_t, st));
+  }
+  PSM_HIP(c, upload(c->vel, velocity_slice, n_buf_xl * n_t * sizeof(float), st));
+  PSM_HIP(c, upload(c->t0, t0_times, n_t * sizeof(float), st));
+  psm::Task tk{contiguous ? (int)crow : 0, 0, (int)n_buf_il, 0};
+  PSM_HIP(c, upload(c->tasks, &tk, sizeof tk, st));
+  if (!contiguous && n_buf_il) PSM_HIP(c, upload(c->rowpos, il_pos, n_buf_il * sizeof(int32_t), st));
+  PSM_ALLOC(c, c->out, n_xl * n_t * sizeof(float));
+
+  PSM_HIP(c, ensure_zero_trace(c, n_t, st));
+  psm::MigArgs a{};
+  a.xp = c->pairs.as<float2>();
+  a.zero_tr = c->zero.as<float2>();
+  a.vel = c->vel.as<float>();
+  a.out = c->out.as<float>();
+  a.t0 = c->t0.as<float>();
+  a.tasks = c->tasks.as<psm::Task>();
+  a.row_pos = (!contiguous && n_buf_il) ? c->rowpos.as<int>() : nullptr;
+  a.counters = nullptr;
+  a.n_t = (int)n_t;
+  a.n_xd = (int)n_buf_xl;
+  a.xd_off = 0;
+  a.n_xv = (int)n_buf_xl;
+  a.xv_off = 0;
+  a.K = (int)n_buf_xl;
+  a.n_out_xl = (int)n_xl;
+  a.cur_xl_start = current_xl_start;
+  a.n_out_il = 1;
+  a.cpos = cpos;
+  a.inv_dt = 1.0f / dt;                      // migration_kernel.cpp:44
+  a.dx = inline_bin_size;
+  a.dy = crossline_bin_size;
+  a.inv_dy = 1.0f / crossline_bin_size;      // :45
+  a.tan2 = tan_theta_sq;
+  a.amp = amp_correction ? 1 : 0;
+  if (n_buf_il == 0) {
+    // no aperture traces: every valid sample sums to zero
+    a.tasks = c->tasks.as<psm::Task>();
+  }
+  PSM_HIP(c, psm::launch_migrate(a, false, st));
+  PSM_HIP(c, hipMemcpyAsync(output_slice, c->out.p, n_xl * n_t * sizeof(float), hipMemcpyDeviceToHost, st));
+  PSM_HIP(c, hipStreamSynchronize(st));
+  return PSM_OK;
+}
+
+int psm_survey_plan(const psm_survey* s, const float* vel_probe, int where, psm_plan* plan) {
+  return plan_impl(nullptr, s, vel_probe, where, plan);
+}
+
+void psm_plan_free(psm_plan* p) {
+  if (!p) return;
+  std::free(p->out_inline_labels);
+  std::free(p->out_crossline_labels);
+  std::free(p->out_to_in_il);
+  std::free(p->window);
+  std::free(p->t0_times);
+  std::memset(p, 0, sizeof *p);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Shared body of psm_migrate_volume / psm_count_pairs (lock held).
+int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float* vel,
+                float* out, int where, bool count, double* p_ap, double* p_g) {
+  if (where != PSM_HOST && where != PSM_DEVICE) return fail(c, PSM_ERR_ARG, "bad `where`");
+  if (!vel || (!count && (!data || !out))) return fail(c, PSM_ERR_ARG, "null data/vel/out pointer");
+  PSM_HIP(c, hipSetDevice(c->device));
+  psm_plan plan;
+  int rc = plan_impl(c, s, vel, where, &plan);
+  if (rc) return rc;
+  struct PlanGuard {
+    psm_plan* p;
+    ~PlanGuard() { psm_plan_free(p); }
+  } guard{&plan};
+  hipStream_t st = c->stream;
+  const size_t n_tr = s->n_il * s->n_xl, nt = s->n_t;
+  const size_t cube = n_tr * nt;
+  const int padx = s->crossline_padding;
+  const size_t K = s->n_xl + 2 * (size_t)padx;
+
+  std::vector<psm::Task> tasks(plan.n_out_il);
+  for (size_t io = 0; io < plan.n_out_il; ++io)
+    tasks[io] = psm::Task{plan.out_to_in_il[io], plan.window[2 * io], plan.window[2 * io + 1],
+                          plan.out_to_in_il[io]};
+  PSM_HIP(c, upload(c->tasks, tasks.data(), tasks.size() * sizeof(psm::Task), st));
+  PSM_HIP(c, upload(c->t0, plan.t0_times, nt * sizeof(float), st));
+
+  const float* d_vel = vel;
+  const float* d_data = data;
+  if (where == PSM_HOST) {
+    PSM_HIP(c, upload(c->vel, vel, cube * sizeof(float), st));
+    d_vel = c->vel.as<float>();
+    if (!count) {
+      PSM_HIP(c, upload(c->raw, data, cube * sizeof(float), st));
+      d_data = c->raw.as<float>();
+    }
+  }
+  PSM_ALLOC(c, c->pairs, std::max<size_t>(cube * sizeof(float2), 8));
+  if (!count) PSM_HIP(c, psm::launch_pack_pairs(d_data, c->pairs.as<float2>(), n_tr, (int)nt, st));
+
+  float* d_out = out;
+  const size_t out_elems = plan.n_out_il * plan.n_out_xl * nt;
+  if (!count && where == PSM_HOST) {
+    PSM_ALLOC(c, c->out, out_elems * sizeof(float));
+    d_out = c->out.as<float>();
+  }
+  PSM_HIP(c, ensure_zero_trace(c, nt, st));
+  psm::MigArgs a{};
+  a.xp = c->pairs.as<float2>();
+  a.zero_tr = c->zero.as<float2>();
+  a.vel = d_vel;
+  a.out = d_out;
+  a.t0 = c->t0.as<float>();
+  a.tasks = c->tasks.as<psm::Task>();
+  a.row_pos = nullptr;
+  a.n_t = (int)nt;
+  a.n_xd = (int)s->n_xl;
+  a.xd_off = padx;
+  a.n_xv = (int)s->n_xl;
+  a.xv_off = padx;
+  a.K = (int)K;
+  a.n_out_xl = (int)plan.n_out_xl;
+  a.cur_xl_start = 0;  // main.cpp:362
+  a.n_out_il = (int)plan.n_out_il;
+  a.cpos = 0;
+  const float dtf = (float)s->dt;  // main.cpp:363
+  a.inv_dt = 1.0f / dtf;
+  a.dx = (float)s->inline_step;
+  a.dy = (float)s->crossline_step;
+  a.inv_dy = 1.0f / a.dy;
+  a.tan2 = plan.tan_theta_sq;
+  a.amp = s->amp_correction ? 1 : 0;
+  if (count) {
+    PSM_ALLOC(c, c->counters, 2 * sizeof(unsigned long long));
+    PSM_HIP(c, hipMemsetAsync(c->counters.p, 0, 2 * sizeof(unsigned long long), st));
+    a.counters = c->counters.as<unsigned long long>();
+    a.out = nullptr;
+    PSM_HIP(c, psm::launch_migrate(a, true, st));
+    unsigned long long h[2] = {0, 0};
+    PSM_HIP(c, hipMemcpyAsync(h, c->counters.p, sizeof h, hipMemcpyDeviceToHost, st));
+    PSM_HIP(c, hipStreamSynchronize(st));
+    if (p_ap) *p_ap = (double)h[0];
+    if (p_g) *p_g = (double)h[1];
+    return PSM_OK;
+  }
+  PSM_HIP(c, psm::launch_migrate(a, false, st));
+  if (where == PSM_HOST) {
+    PSM_HIP(c, hipMemcpyAsync(out, d_out, out_elems * sizeof(float), hipMemcpyDeviceToHost, st));
+    PSM_HIP(c, hipStreamSynchronize(st));
+  }
+  return PSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psm_migrate_volume(psm_ctx* c, const psm_survey* s, const float* data, const float* vel,
+                       float* out, int where) {
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return volume_impl(c, s, data, vel, out, where, false, nullptr, nullptr);
+}
+
+int psm_count_pairs(psm_ctx* c, const psm_survey* s, const float* vel, int where, double* p_ap,
+                    double* p_g) {
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return volume_impl(c, s, nullptr, vel, nullptr, where, true, p_ap, p_g);
+}
+
+}  // extern "C"

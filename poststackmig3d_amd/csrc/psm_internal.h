@@ -1,0 +1,45 @@
+// psm_internal.h -- shared declarations between the C ABI (psm_capi.cpp) and
+// the gfx950 kernels (psm_kernels.hip).  Not installed; callers use psm.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace psm {
+
+// One output inline of work: the centre row (in data-row index space), the
+// data-row window [b0, b1) it sums over, and the velocity row it reads.
+struct Task {
+  int c, b0, b1, vrow;
+};
+
+// Arguments of the forward (migration) kernel.  Crossline coordinates are
+// "buffer" coordinates as in migration_kernel.cpp: the data crosslines occupy
+// [xd_off, xd_off + n_xd), the velocity crosslines [xv_off, xv_off + n_xv),
+// and K is the buffer width n_buf_xl that clamps the crossline window.
+struct MigArgs {
+  const float2* xp;        // traces in pair layout [n_rows][n_xd][n_t]: (x[i], x[i+1]-x[i]),
+                           // sample n_t-1 holds (0, 0)
+  const float2* zero_tr;   // one trace of n_t (0, 0) pairs, read for absent partners
+  const float* vel;        // velocity rows [n_vrows][n_xv][n_t]
+  float* out;              // [n_out_il][n_out_xl][n_t]
+  const float* t0;         // [n_t]
+  const Task* tasks;       // [n_out_il]
+  const int* row_pos;      // generic-row mode: position of each data row; nullptr = contiguous
+  unsigned long long* counters;  // count mode: {p_ap, p_g}
+  int n_t, n_xd, xd_off, n_xv, xv_off, K;
+  int n_out_xl, cur_xl_start, n_out_il;
+  int cpos;                // generic-row mode: position of the current inline
+  float inv_dt, dx, dy, inv_dy, tan2;
+  int amp;
+};
+
+// Launchers (psm_kernels.hip).  All enqueue on `stream` and return the HIP
+// status of the launch.
+hipError_t launch_pack_pairs(const float* x, float2* xp, size_t n_traces, int n_t,
+                             hipStream_t stream);
+hipError_t launch_migrate(const MigArgs& a, bool count_only, hipStream_t stream);
+
+}  // namespace psm

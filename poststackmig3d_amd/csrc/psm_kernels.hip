@@ -1,0 +1,325 @@
+// psm_kernels.hip -- gfx950 kernels of the Kirchhoff post-stack 3D time
+// migration (diffraction summation).
+//
+// Forward operator, per output sample (image point j at time t0) of one
+// output inline, as MigrationKernel::kirchhoffKernel3DWithPadding
+// (migration_kernel.cpp:17-383) defines it:
+//
+//   out = sum over aperture traces (b, k) with r^2 <= R^2 and i < n_t-1 of
+//         w * lerp(x_bk, q),   q = tau/dt, i = trunc(q), f = q - i,
+//         tau = sqrt(t0^2 + 4 r^2 / v^2),  R^2 = (v t0 / 2)^2 tan^2(theta),
+//         w = 1/(tau v^2) when amp_correction, else 1,
+//   r^2 = (db dx)^2 + (dk dy)^2, v = Vrms at the image point.
+//
+// Mapping to CDNA4:
+//   * one wavefront = one output trace x 64 consecutive t0 samples (lane =
+//     sample), so the aperture loop is wave-uniform and r^2 lives in scalar
+//     registers; the 64 gathers of one aperture trace touch one contiguous
+//     stretch of that trace (dtau/dt0 <= 1);
+//   * the aperture is walked in (|db|, |dk|) order and each (|db|, |dk|)
+//     shares tau, i, f and the weight between the up-to-four traces
+//     (c +- db, kj +- dk) that have the same r^2 -- the travel time depends
+//     only on r^2 and on the image point's velocity;
+//   * traces are stored in a pair layout (x[i], x[i+1]-x[i]) so that one
+//     8-byte load and one FMA give the linear interpolation of
+//     migration_kernel.cpp:544-550, and the last sample holds (0,0) so the
+//     bound i < n_t-1 (:298) becomes a clamp instead of a mask;
+//   * workgroups are ordered latest t0 tile first (work grows as t0^2) and
+//     striped so that each XCD takes a contiguous slab of output traces of
+//     every tile (its own L2 sees neighbouring apertures).
+//
+// Arithmetic follows the reference's compiled order: r^2 = fma(dk dy, dk dy,
+// dx^2) and tau^2 = fma(4/v^2, r^2, t0^2) are single-rounding FMAs (the
+// -ffast-math build contracts them), tau is the correctly rounded square root,
+// q = tau * (1/dt) with 1/dt rounded once.  The file is compiled with
+// -ffp-contract=off so that these FMAs are exactly the ones written here.
+#include <hip/hip_runtime.h>
+
+#include "psm_internal.h"
+
+namespace psm {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kXcds = 8;
+
+// Correctly rounded square root for positive normal x: v_sqrt_f32 is within
+// one ulp, and the neighbour whose square brackets x is chosen by the sign of
+// the fused residuals x - s_dn*s and x - s_up*s.
+__device__ __forceinline__ float sqrt_rn(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const int si = __float_as_int(s);
+  const float s_dn = __int_as_float(si - 1);
+  const float s_up = __int_as_float(si + 1);
+  const float r_dn = __builtin_fmaf(-s_dn, s, x);
+  const float r_up = __builtin_fmaf(-s_up, s, x);
+  float r = (r_dn <= 0.0f) ? s_dn : s;
+  r = (r_up > 0.0f) ? s_up : r;
+  return r;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Per-lane image-point parameters (migration_kernel.cpp:115-144), evaluated
+// for the crossline `src` whose velocity the lane uses.
+struct LaneParams {
+  float t0sq, a, R2, v2;
+  int wlo, whi;
+};
+
+// Sums the aperture of one (output trace, t0 tile).  WINDOW = some lane uses
+// parameters of an earlier crossline whose crossline window can cut the
+// aperture (sticky samples of migration_kernel.cpp:64-152), so each gather is
+// also checked against that window.  COUNT = count pairs instead of summing.
+// All loop bounds and trace pointers are wave-uniform (scalar registers);
+// absent partners read a trace of zeros so the four gathers of a group are
+// issued back to back without branches.
+template <bool GENERIC_ROWS, bool WINDOW, bool COUNT>
+__device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, int kj,
+                                             const LaneParams& p, float R2max,
+                                             float& acc, unsigned& cap, unsigned& cg) {
+  const int n_t = a.n_t;
+  const int nt1 = n_t - 1;
+  const int xd_lo = a.xd_off, xd_hi = a.xd_off + a.n_xd;
+  const size_t row_stride = (size_t)a.n_xd * n_t;
+  const size_t tr_bytes = (size_t)n_t * sizeof(float2);
+  const char* const zero = reinterpret_cast<const char*>(a.zero_tr);
+  // largest crossline offset that can hold data
+  const int dk_lim = max(kj - xd_lo, xd_hi - 1 - kj);
+
+  auto row_sweep = [&](const char* rowp, const char* rowm, float d2) {
+    const float rem = R2max - d2;
+    int kmax = (int)(__builtin_sqrtf(rem) * a.inv_dy) + 2;
+    kmax = uniform_i(min(kmax, dk_lim));
+    for (int dk = 0; dk <= kmax; ++dk) {
+      const float dyv = (float)dk * a.dy;
+      const float r2 = uniform_f(__builtin_fmaf(dyv, dyv, d2));
+      if (r2 > R2max) break;
+      const int kp = kj + dk, km = kj - dk;
+      const bool has_kp = kp >= xd_lo && kp < xd_hi;
+      const bool has_km = dk > 0 && km >= xd_lo && km < xd_hi;
+      if (!has_kp && !has_km) continue;
+      const bool in_mask = r2 <= p.R2;
+      const float tsq = __builtin_fmaf(p.a, r2, p.t0sq);
+      const float tau = sqrt_rn(tsq);
+      const float q = tau * a.inv_dt;
+      const int i = (int)q;
+      bool okp = true, okm = true;
+      if (WINDOW) {
+        okp = kp >= p.wlo && kp < p.whi;
+        okm = km >= p.wlo && km < p.whi;
+      }
+      if constexpr (COUNT) {
+        const unsigned nrow = (rowp ? 1u : 0u) + (rowm ? 1u : 0u);
+        const unsigned n = nrow * ((has_kp && okp ? 1u : 0u) + (has_km && okm ? 1u : 0u));
+        cap += in_mask ? n : 0u;
+        cg += (in_mask && i < nt1) ? n : 0u;
+      } else {
+        const char* t0p = (rowp && has_kp) ? rowp + (size_t)(kp - xd_lo) * tr_bytes : zero;
+        const char* t0m = (rowp && has_km) ? rowp + (size_t)(km - xd_lo) * tr_bytes : zero;
+        const char* t1p = (rowm && has_kp) ? rowm + (size_t)(kp - xd_lo) * tr_bytes : zero;
+        const char* t1m = (rowm && has_km) ? rowm + (size_t)(km - xd_lo) * tr_bytes : zero;
+        const unsigned off = (unsigned)min(i, nt1) * (unsigned)sizeof(float2);
+        const float2 x0 = *reinterpret_cast<const float2*>(t0p + off);
+        const float2 x1 = *reinterpret_cast<const float2*>(t0m + off);
+        const float2 x2 = *reinterpret_cast<const float2*>(t1p + off);
+        const float2 x3 = *reinterpret_cast<const float2*>(t1m + off);
+        const float f = __builtin_amdgcn_fractf(q);
+        float w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+        w = in_mask ? w : 0.0f;
+        float l0 = __builtin_fmaf(f, x0.y, x0.x);
+        float l1 = __builtin_fmaf(f, x1.y, x1.x);
+        float l2 = __builtin_fmaf(f, x2.y, x2.x);
+        float l3 = __builtin_fmaf(f, x3.y, x3.x);
+        if (WINDOW) {
+          l0 = okp ? l0 : 0.0f;
+          l2 = okp ? l2 : 0.0f;
+          l1 = okm ? l1 : 0.0f;
+          l3 = okm ? l3 : 0.0f;
+        }
+        acc = __builtin_fmaf((l0 + l1) + (l2 + l3), w, acc);
+      }
+    }
+  };
+
+  if constexpr (GENERIC_ROWS) {
+    for (int b = tk.b0; b < tk.b1; ++b) {
+      const float dxv = (float)(a.row_pos[b] - a.cpos) * a.dx;
+      const float d2 = uniform_f(dxv * dxv);
+      if (d2 > R2max) continue;
+      row_sweep(reinterpret_cast<const char*>(a.xp + (size_t)b * row_stride), nullptr, d2);
+    }
+  } else {
+    const int c = tk.c;
+    int dmax = (int)(__builtin_sqrtf(R2max) / a.dx) + 2;
+    dmax = uniform_i(min(dmax, max(tk.b1 - 1 - c, c - tk.b0)));
+    for (int d = 0; d <= dmax; ++d) {
+      const float dxv = (float)d * a.dx;
+      const float d2 = uniform_f(dxv * dxv);
+      if (d2 > R2max) break;
+      const int rp = c + d, rm = c - d;
+      const bool hp = rp >= tk.b0 && rp < tk.b1;
+      const bool hm = d > 0 && rm >= tk.b0 && rm < tk.b1;
+      if (!hp && !hm) continue;
+      row_sweep(hp ? reinterpret_cast<const char*>(a.xp + (size_t)rp * row_stride) : nullptr,
+                hm ? reinterpret_cast<const char*>(a.xp + (size_t)rm * row_stride) : nullptr, d2);
+    }
+  }
+}
+
+template <bool GENERIC_ROWS, bool COUNT>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void migrate_kernel(MigArgs a, int n_tiles,
+                                                                          int n_jgroups,
+                                                                          int per_tile_pad) {
+  // ---- block -> (t0 tile, output inline, crossline group), heavy tiles first
+  const int g = blockIdx.x;
+  const int tile = n_tiles - 1 - g / per_tile_pad;
+  const int rem = g % per_tile_pad;
+  const int spatial = (rem % kXcds) * (per_tile_pad / kXcds) + rem / kXcds;
+  if (spatial >= a.n_out_il * n_jgroups) return;
+  const int io = spatial / n_jgroups;
+  const int j = (spatial % n_jgroups) * kWavesPerBlock + uniform_i((int)(threadIdx.x / kWave));
+  if (j >= a.n_out_xl) return;
+  const int lane = threadIdx.x % kWave;
+  const int s = tile * kWave + lane;
+  const bool active = s < a.n_t;
+  const int kj = a.cur_xl_start + j;
+  float* orow = a.out + ((size_t)io * a.n_out_xl + j) * a.n_t;
+
+  if (kj < 0 || kj >= a.K) {  // migration_kernel.cpp:79-85
+    if (!COUNT && active) orow[s] = 0.0f;
+    return;
+  }
+  const Task tk = a.tasks[io];
+  const float* vrowp = a.vel + (size_t)tk.vrow * a.n_xv * a.n_t;
+
+  // ---- parameter source: this crossline, or the last earlier crossline of
+  // the call whose sample was valid (sticky caches, migration_kernel.cpp:64-152)
+  const float t0 = active ? a.t0[s] : 0.0f;
+  int src = -1;
+  float v = 0.0f;
+  if (active && t0 >= 1e-5f) {
+    const int kmin = max(max(0, a.cur_xl_start), a.xv_off);
+    for (int kk = min(kj, a.xv_off + a.n_xv - 1); kk >= kmin; --kk) {
+      const float vv = vrowp[(size_t)(kk - a.xv_off) * a.n_t + s];
+      if (vv >= 1e-5f) {
+        src = kk;
+        v = vv;
+        break;
+      }
+    }
+  }
+  LaneParams p{0.0f, 0.0f, -1.0f, 1.0f, 0, 0};
+  bool needs_window = false;
+  if (src >= 0) {
+    p.t0sq = t0 * t0;
+    const float inv_v2 = 1.0f / (v * v + 1e-10f);
+    p.a = 4.0f * inv_v2;
+    const float vth = v * t0 * 0.5f;
+    p.R2 = vth * vth * a.tan2;
+    const int w = (int)(sqrt_rn(p.R2) * a.inv_dy) + 1;
+    p.wlo = max(0, src - w);
+    p.whi = min(a.K, src + w + 1);
+    p.v2 = v * v;
+    if (src != kj) {
+      // crosslines this lane could reach around kj, clipped to the data
+      const int kr = (int)(sqrt_rn(p.R2) * a.inv_dy) + 1;
+      const int lo_need = max(a.xd_off, kj - kr);
+      const int hi_need = min(a.xd_off + a.n_xd, kj + kr + 1);
+      needs_window = (p.wlo > lo_need) || (p.whi < hi_need);
+    }
+  }
+  const float R2max = uniform_f(wave_max(p.R2));
+  float acc = 0.0f;
+  unsigned cap = 0, cg = 0;
+  if (R2max >= 0.0f) {
+    if (__any(needs_window))
+      sum_aperture<GENERIC_ROWS, true, COUNT>(a, tk, kj, p, R2max, acc, cap, cg);
+    else
+      sum_aperture<GENERIC_ROWS, false, COUNT>(a, tk, kj, p, R2max, acc, cap, cg);
+  }
+  if constexpr (COUNT) {
+    const unsigned tap = wave_sum_u32(cap), tg = wave_sum_u32(cg);
+    if (lane == 0) {
+      atomicAdd(&a.counters[0], (unsigned long long)tap);
+      atomicAdd(&a.counters[1], (unsigned long long)tg);
+    }
+  } else {
+    if (active) orow[s] = (src >= 0) ? acc : 0.0f;  // :148-151, :375
+  }
+}
+
+__global__ void pack_pairs_kernel(const float* __restrict__ x, float2* __restrict__ xp,
+                                  size_t n, int n_t) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += stride) {
+    const int i = (int)(idx % (size_t)n_t);
+    float2 pr = make_float2(0.0f, 0.0f);
+    if (i < n_t - 1) {
+      const float lo = x[idx];
+      pr = make_float2(lo, x[idx + 1] - lo);
+    }
+    xp[idx] = pr;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_pack_pairs(const float* x, float2* xp, size_t n_traces, int n_t,
+                             hipStream_t stream) {
+  const size_t n = n_traces * (size_t)n_t;
+  if (n == 0) return hipSuccess;
+  const int threads = 256;
+  size_t blocks = (n + threads - 1) / threads;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(pack_pairs_kernel, dim3((unsigned)blocks), dim3(threads), 0, stream, x, xp,
+                     n, n_t);
+  return hipGetLastError();
+}
+
+hipError_t launch_migrate(const MigArgs& a, bool count_only, hipStream_t stream) {
+  if (a.n_out_il <= 0 || a.n_out_xl <= 0 || a.n_t <= 0) return hipSuccess;
+  const int n_tiles = (a.n_t + kWave - 1) / kWave;
+  const int n_jgroups = (a.n_out_xl + kWavesPerBlock - 1) / kWavesPerBlock;
+  const long long per_tile = (long long)a.n_out_il * n_jgroups;
+  const long long per_tile_pad = (per_tile + kXcds - 1) / kXcds * kXcds;
+  const long long blocks = per_tile_pad * n_tiles;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)blocks), block(kWave * kWavesPerBlock);
+  const bool generic = a.row_pos != nullptr;
+  if (count_only) {
+    if (generic)
+      hipLaunchKernelGGL((migrate_kernel<true, true>), grid, block, 0, stream, a, n_tiles,
+                         n_jgroups, (int)per_tile_pad);
+    else
+      hipLaunchKernelGGL((migrate_kernel<false, true>), grid, block, 0, stream, a, n_tiles,
+                         n_jgroups, (int)per_tile_pad);
+  } else {
+    if (generic)
+      hipLaunchKernelGGL((migrate_kernel<true, false>), grid, block, 0, stream, a, n_tiles,
+                         n_jgroups, (int)per_tile_pad);
+    else
+      hipLaunchKernelGGL((migrate_kernel<false, false>), grid, block, 0, stream, a, n_tiles,
+                         n_jgroups, (int)per_tile_pad);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace psm
