@@ -50,8 +50,9 @@ const char* psm_version(void);
 int psm_create(psm_ctx** ctx, int device);
 void psm_destroy(psm_ctx* ctx);
 const char* psm_last_error(const psm_ctx* ctx);
-/* Use an existing HIP stream (hipStream_t passed as void*, NULL = the
- * context's own stream) for every later call on this context. */
+/* Use an existing HIP stream (hipStream_t passed as void*; NULL = the null
+ * (default) stream) for every later call on this context.  A new context
+ * uses a stream of its own. */
 int psm_set_stream(psm_ctx* ctx, void* stream);
 int psm_synchronize(psm_ctx* ctx);
 
@@ -134,11 +135,25 @@ int psm_migrate_volume(psm_ctx* ctx, const psm_survey* survey,
                        const float* data, const float* vel, float* out,
                        int where);
 
-/* Pair counts of the work unit for a survey: p_ap = (output sample, real
- * aperture trace) pairs inside the aperture mask, p_g = those that also
- * gather (i < n_t - 1).  vel is host or device per `where`. */
+/* Pair counts of the work unit for a survey (no reference counterpart):
+ * p_ap = (output sample, real aperture trace) pairs inside the aperture
+ * mask, p_g = those that also gather (i < n_t - 1).  ap_image / g_image
+ * (nullable, [n_out_il][n_out_xl][n_t]) receive the per-sample counts.
+ * vel and the images are host or device memory per `where`. */
 int psm_count_pairs(psm_ctx* ctx, const psm_survey* survey, const float* vel,
-                    int where, double* p_ap, double* p_g);
+                    int where, double* p_ap, double* p_g,
+                    float* ap_image, float* g_image);
+
+/* Kernel-level timing (no reference counterpart): when enabled, each
+ * psm_migrate_volume records HIP events on the context's stream around the
+ * trace packing kernel and the migration kernel; psm_last_timing waits for
+ * the last call and returns the two durations in milliseconds. */
+typedef struct psm_timing {
+    float pack_ms;
+    float migrate_ms;
+} psm_timing;
+int psm_enable_timing(psm_ctx* ctx, int on);
+int psm_last_timing(psm_ctx* ctx, psm_timing* timing);
 
 #ifdef __cplusplus
 }

@@ -590,3 +590,49 @@ PSMO_API int psmo_count_pairs(const float* vel, const psmo_volume* g,
     free(ill); free(oll);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Bounded sample of the volume loop (CPU baseline, port kind): for each    */
+/* listed output inline, the f32 restatement computes only the output       */
+/* crosslines [xl0, xl0 + m) (current_xl_start = xl0).  Regular labels.     */
+/* out: [n_rows][m][n_t].                                                   */
+/* ------------------------------------------------------------------------ */
+PSMO_API int psmo_migrate_sample(const float* data, const float* vel, const psmo_volume* g,
+                                 const int32_t* rows, size_t n_rows, int xl0, size_t m,
+                                 float* out, int nthreads)
+{
+    const size_t n_il = g->n_il, n_xl = g->n_xl, n_t = g->n_t;
+    int32_t* ill = (int32_t*)malloc(sizeof(int32_t) * n_il);
+    for (size_t i = 0; i < n_il; ++i) ill[i] = (int32_t)(i + 1);
+    size_t nol = psmo_output_labels(ill, n_il, g->pad_il, NULL);
+    int32_t* oll = (int32_t*)malloc(sizeof(int32_t) * (nol ? nol : 1));
+    psmo_output_labels(ill, n_il, g->pad_il, oll);
+    const float vmax = psmo_estimate_max_velocity(vel, n_il, n_xl, n_t);
+    float t2; int ap;
+    psmo_aperture(g->angle_deg, vmax, n_t, g->dt, g->il_step, &t2, &ap);
+    float* t0 = (float*)malloc(sizeof(float) * n_t);
+    for (size_t i = 0; i < n_t; ++i) t0[i] = (float)((double)i * g->dt);
+    const size_t K = n_xl + 2 * (size_t)g->pad_xl;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t r = 0; r < n_rows; ++r) {
+        const size_t c = psmo_map_output_inline(ill, n_il, oll[rows[r]]);
+        size_t wmax = 2 * (size_t)(ap + g->pad_il) + 1;
+        if (wmax > n_il) wmax = n_il;
+        float* buf = (float*)malloc(sizeof(float) * wmax * K * n_t);
+        float* vs = (float*)malloc(sizeof(float) * K * n_t);
+        int32_t* pos = (int32_t*)malloc(sizeof(int32_t) * wmax);
+        size_t nb; int32_t cur;
+        build_window(data, vel, g, c, ap, buf, vs, pos, &nb, &cur);
+        psmo_kernel_inline_f32(buf, nb, K, n_t, vs, out + r * m * n_t, m, pos, nb, cur, xl0,
+                               (float)g->dt, t0, t2, (float)g->il_step, (float)g->xl_step,
+                               g->amp);
+        free(buf); free(vs); free(pos);
+    }
+    free(ill); free(oll); free(t0);
+    return 0;
+}

@@ -80,6 +80,8 @@ def lib():
                                          C.c_size_t, C.c_size_t, _ip, C.c_size_t, C.c_int]
         L.psmo_adjoint_volume_f64.argtypes = [_dp, _fp, C.POINTER(Volume), _dp,
                                               C.c_size_t, C.c_size_t]
+        L.psmo_migrate_sample.argtypes = [_fp, _fp, C.POINTER(Volume), _ip, C.c_size_t, C.c_int,
+                                          C.c_size_t, _fp, C.c_int]
         L.psmo_count_pairs.argtypes = [_fp, C.POINTER(Volume), _ip, C.c_size_t, _dp, _dp, C.c_int]
         _lib = L
     return _lib
@@ -99,6 +101,10 @@ def ref():
             _fp, C.c_size_t, C.c_size_t, C.c_size_t, _fp, _fp, C.c_size_t, _ip,
             C.c_size_t, C.c_int32, C.c_int32, C.c_float, _fp, C.c_float, C.c_float,
             C.c_float, C.c_int]
+        L.psmr_migrate_sample.argtypes = [
+            _fp, _fp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double, C.c_double,
+            C.c_double, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, _ip, _ip, C.c_size_t,
+            C.c_int, C.c_size_t, _fp, C.c_int]
         L.psmr_migrate_volume.argtypes = [
             _fp, _fp, C.c_size_t, C.c_size_t, C.c_size_t, C.c_double, C.c_double,
             C.c_double, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, _ip, _fp,
@@ -205,6 +211,39 @@ def ref_migrate_volume(data, vel, dt, dx, dy, pad_il=0, pad_xl=0, angle=30.0, am
     ref().psmr_migrate_volume(_f(data), _f(vel), n_il, n_xl, n_t, dt, dx, dy, pad_il, pad_xl,
                               t2, ap, int(amp), _i(c_of), _f(out), shp[0], shp[1], _i(rws),
                               0 if rws is None else rws.size, nthreads)
+    return out
+
+
+def ref_migrate_sample(data, vel, dt, dx, dy, pad_il, pad_xl, angle, amp, rows, xl0, m,
+                       nthreads=0):
+    """Reference kernel on a bounded sample: output inlines `rows`, output
+    crosslines [xl0, xl0+m) of each (oracle/_ref).  Returns [len(rows)][m][n_t]."""
+    data = np.ascontiguousarray(data, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    n_il, n_xl, n_t = data.shape
+    vmax = estimate_max_velocity(vel)
+    t2, ap = aperture(angle, vmax, n_t, dt, dx)
+    ill = np.arange(1, n_il + 1, dtype=np.int32)
+    oll = output_labels(ill, pad_il)
+    c_of = np.array([map_output_inline(ill, l) for l in oll], np.int32)
+    rws = np.ascontiguousarray(rows, np.int32)
+    out = np.zeros((rws.size, m, n_t), np.float32)
+    ref().psmr_migrate_sample(_f(data), _f(vel), n_il, n_xl, n_t, dt, dx, dy, pad_il, pad_xl,
+                              t2, ap, int(amp), _i(c_of), _i(rws), rws.size, int(xl0), int(m),
+                              _f(out), nthreads)
+    return out
+
+
+def migrate_sample(data, vel, dt, dx, dy, pad_il, pad_xl, angle, amp, rows, xl0, m,
+                   nthreads=0):
+    """Oracle restatement on the same bounded sample as ref_migrate_sample."""
+    data = np.ascontiguousarray(data, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    g = _vol(data, dt, dx, dy, pad_il, pad_xl, angle, amp)
+    rws = np.ascontiguousarray(rows, np.int32)
+    out = np.zeros((rws.size, m, data.shape[2]), np.float32)
+    lib().psmo_migrate_sample(_f(data), _f(vel), C.byref(g), _i(rws), rws.size, int(xl0), int(m),
+                              _f(out), nthreads)
     return out
 
 

@@ -43,6 +43,11 @@ class Plan(C.Structure):
     ]
 
 
+class Timing(C.Structure):
+    """psm_timing (include/psm.h)."""
+    _fields_ = [("pack_ms", C.c_float), ("migrate_ms", C.c_float)]
+
+
 class PsmError(RuntimeError):
     pass
 
@@ -67,8 +72,24 @@ SIGNATURES = [
     ("psm_migrate_volume", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_int]),
     ("psm_count_pairs", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p, C.c_int,
-                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p,
+                                  C.c_void_p]),
+    ("psm_enable_timing", C.c_int, [C.c_void_p, C.c_int]),
+    ("psm_last_timing", C.c_int, [C.c_void_p, C.POINTER(Timing)]),
 ]
+
+
+def _share_torch_hip_runtime():
+    """PyTorch ships its own libamdhip64.so with the same soname
+    (libamdhip64.so.7) as /opt/rocm's.  Loading torch first makes the dynamic
+    linker resolve this library's HIP dependency to that already-loaded
+    runtime, so the process holds one HIP runtime and torch tensors, streams
+    and this library's calls share it.  (Loaded the other way round, torch
+    would load a second runtime that cannot see the GPU.)"""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def load():
@@ -76,6 +97,7 @@ def load():
     global _lib
     with _lock:
         if _lib is None:
+            _share_torch_hip_runtime()
             if not os.path.exists(LIB_PATH):
                 raise PsmError(
                     f"libpsm_hip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
@@ -107,10 +129,20 @@ class Context:
             raise PsmError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
     def set_stream(self, stream_ptr: int | None):
+        """Runs later calls on this HIP stream (0/None = the null stream)."""
         self.check(self.lib.psm_set_stream(self.h, C.c_void_p(stream_ptr or 0)), "psm_set_stream")
 
     def synchronize(self):
         self.check(self.lib.psm_synchronize(self.h), "psm_synchronize")
+
+    def enable_timing(self, on: bool = True):
+        self.check(self.lib.psm_enable_timing(self.h, 1 if on else 0), "psm_enable_timing")
+
+    def last_timing(self):
+        """(pack_ms, migrate_ms) of the last volume call (HIP events)."""
+        t = Timing()
+        self.check(self.lib.psm_last_timing(self.h, C.byref(t)), "psm_last_timing")
+        return t.pack_ms, t.migrate_ms
 
     def close(self):
         if getattr(self, "h", None):

@@ -53,6 +53,10 @@ struct psm_ctx {
   std::string err;
   DevBuf raw, pairs, vel, out, t0, tasks, rowpos, counters, zero;
   size_t zero_nt = 0;
+  // kernel-level timing of the last volume call (psm_enable_timing)
+  bool timing = false;
+  bool timed = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -264,6 +268,8 @@ void psm_destroy(psm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->raw, &c->pairs, &c->vel, &c->out, &c->t0, &c->tasks, &c->rowpos, &c->counters, &c->zero})
     b->release();
+  for (hipEvent_t& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
 }
@@ -273,7 +279,7 @@ const char* psm_last_error(const psm_ctx* c) { return c ? c->err.c_str() : "no c
 int psm_set_stream(psm_ctx* c, void* stream) {
   if (!c) return PSM_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
-  c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+  c->stream = static_cast<hipStream_t>(stream);
   return PSM_OK;
 }
 
@@ -388,7 +394,8 @@ namespace {
 
 // Shared body of psm_migrate_volume / psm_count_pairs (lock held).
 int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float* vel,
-                float* out, int where, bool count, double* p_ap, double* p_g) {
+                float* out, int where, bool count, double* p_ap, double* p_g,
+                float* ap_image = nullptr, float* g_image = nullptr) {
   if (where != PSM_HOST && where != PSM_DEVICE) return fail(c, PSM_ERR_ARG, "bad `where`");
   if (!vel || (!count && (!data || !out))) return fail(c, PSM_ERR_ARG, "null data/vel/out pointer");
   PSM_HIP(c, hipSetDevice(c->device));
@@ -422,8 +429,15 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
       d_data = c->raw.as<float>();
     }
   }
-  PSM_ALLOC(c, c->pairs, std::max<size_t>(cube * sizeof(float2), 8));
+  if (!count) PSM_ALLOC(c, c->pairs, std::max<size_t>(cube * sizeof(float2), 8));
+  c->timed = false;
+  if (c->timing && !count) {
+    for (hipEvent_t& e : c->ev)
+      if (!e) PSM_HIP(c, hipEventCreate(&e));
+    PSM_HIP(c, hipEventRecord(c->ev[0], st));
+  }
   if (!count) PSM_HIP(c, psm::launch_pack_pairs(d_data, c->pairs.as<float2>(), n_tr, (int)nt, st));
+  if (c->timing && !count) PSM_HIP(c, hipEventRecord(c->ev[1], st));
 
   float* d_out = out;
   const size_t out_elems = plan.n_out_il * plan.n_out_xl * nt;
@@ -462,15 +476,36 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
     PSM_HIP(c, hipMemsetAsync(c->counters.p, 0, 2 * sizeof(unsigned long long), st));
     a.counters = c->counters.as<unsigned long long>();
     a.out = nullptr;
+    a.out_g = nullptr;
+    if (ap_image || g_image) {
+      if (where == PSM_DEVICE) {
+        a.out = ap_image;
+        a.out_g = g_image;
+      } else {
+        PSM_ALLOC(c, c->out, 2 * out_elems * sizeof(float));
+        a.out = ap_image ? c->out.as<float>() : nullptr;
+        a.out_g = g_image ? c->out.as<float>() + out_elems : nullptr;
+      }
+    }
     PSM_HIP(c, psm::launch_migrate(a, true, st));
     unsigned long long h[2] = {0, 0};
     PSM_HIP(c, hipMemcpyAsync(h, c->counters.p, sizeof h, hipMemcpyDeviceToHost, st));
+    if (where == PSM_HOST) {
+      if (ap_image)
+        PSM_HIP(c, hipMemcpyAsync(ap_image, a.out, out_elems * sizeof(float), hipMemcpyDeviceToHost, st));
+      if (g_image)
+        PSM_HIP(c, hipMemcpyAsync(g_image, a.out_g, out_elems * sizeof(float), hipMemcpyDeviceToHost, st));
+    }
     PSM_HIP(c, hipStreamSynchronize(st));
     if (p_ap) *p_ap = (double)h[0];
     if (p_g) *p_g = (double)h[1];
     return PSM_OK;
   }
   PSM_HIP(c, psm::launch_migrate(a, false, st));
+  if (c->timing) {
+    PSM_HIP(c, hipEventRecord(c->ev[2], st));
+    c->timed = true;
+  }
   if (where == PSM_HOST) {
     PSM_HIP(c, hipMemcpyAsync(out, d_out, out_elems * sizeof(float), hipMemcpyDeviceToHost, st));
     PSM_HIP(c, hipStreamSynchronize(st));
@@ -490,10 +525,28 @@ int psm_migrate_volume(psm_ctx* c, const psm_survey* s, const float* data, const
 }
 
 int psm_count_pairs(psm_ctx* c, const psm_survey* s, const float* vel, int where, double* p_ap,
-                    double* p_g) {
+                    double* p_g, float* ap_image, float* g_image) {
   if (!c) return PSM_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
-  return volume_impl(c, s, nullptr, vel, nullptr, where, true, p_ap, p_g);
+  return volume_impl(c, s, nullptr, vel, nullptr, where, true, p_ap, p_g, ap_image, g_image);
+}
+
+int psm_enable_timing(psm_ctx* c, int on) {
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->timing = on != 0;
+  return PSM_OK;
+}
+
+int psm_last_timing(psm_ctx* c, psm_timing* t) {
+  if (!c || !t) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!c->timed) return fail(c, PSM_ERR_ARG, "no timed volume call on this context");
+  PSM_HIP(c, hipSetDevice(c->device));
+  PSM_HIP(c, hipEventSynchronize(c->ev[2]));
+  PSM_HIP(c, hipEventElapsedTime(&t->pack_ms, c->ev[0], c->ev[1]));
+  PSM_HIP(c, hipEventElapsedTime(&t->migrate_ms, c->ev[1], c->ev[2]));
+  return PSM_OK;
 }
 
 }  // extern "C"

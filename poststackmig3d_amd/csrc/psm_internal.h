@@ -29,6 +29,7 @@ struct MigArgs {
   const Task* tasks;       // [n_out_il]
   const int* row_pos;      // generic-row mode: position of each data row; nullptr = contiguous
   unsigned long long* counters;  // count mode: {p_ap, p_g}
+  float* out_g;            // count mode: per-sample P_g image (nullable; `out` holds P_ap)
   int n_t, n_xd, xd_off, n_xv, xv_off, K;
   int n_out_xl, cur_xl_start, n_out_il;
   int cpos;                // generic-row mode: position of the current inline

@@ -104,7 +104,8 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
   // largest crossline offset that can hold data
   const int dk_lim = max(kj - xd_lo, xd_hi - 1 - kj);
 
-  auto row_sweep = [&](const char* rowp, const char* rowm, float d2) {
+  // rowp / rowm: the rows at +db and -db (present when hp / hm)
+  auto row_sweep = [&](const char* rowp, bool hp, const char* rowm, bool hm, float d2) {
     const float rem = R2max - d2;
     int kmax = (int)(__builtin_sqrtf(rem) * a.inv_dy) + 2;
     kmax = uniform_i(min(kmax, dk_lim));
@@ -127,15 +128,15 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
         okm = km >= p.wlo && km < p.whi;
       }
       if constexpr (COUNT) {
-        const unsigned nrow = (rowp ? 1u : 0u) + (rowm ? 1u : 0u);
+        const unsigned nrow = (hp ? 1u : 0u) + (hm ? 1u : 0u);
         const unsigned n = nrow * ((has_kp && okp ? 1u : 0u) + (has_km && okm ? 1u : 0u));
         cap += in_mask ? n : 0u;
         cg += (in_mask && i < nt1) ? n : 0u;
       } else {
-        const char* t0p = (rowp && has_kp) ? rowp + (size_t)(kp - xd_lo) * tr_bytes : zero;
-        const char* t0m = (rowp && has_km) ? rowp + (size_t)(km - xd_lo) * tr_bytes : zero;
-        const char* t1p = (rowm && has_kp) ? rowm + (size_t)(kp - xd_lo) * tr_bytes : zero;
-        const char* t1m = (rowm && has_km) ? rowm + (size_t)(km - xd_lo) * tr_bytes : zero;
+        const char* t0p = (hp && has_kp) ? rowp + (size_t)(kp - xd_lo) * tr_bytes : zero;
+        const char* t0m = (hp && has_km) ? rowp + (size_t)(km - xd_lo) * tr_bytes : zero;
+        const char* t1p = (hm && has_kp) ? rowm + (size_t)(kp - xd_lo) * tr_bytes : zero;
+        const char* t1m = (hm && has_km) ? rowm + (size_t)(km - xd_lo) * tr_bytes : zero;
         const unsigned off = (unsigned)min(i, nt1) * (unsigned)sizeof(float2);
         const float2 x0 = *reinterpret_cast<const float2*>(t0p + off);
         const float2 x1 = *reinterpret_cast<const float2*>(t0m + off);
@@ -164,7 +165,8 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
       const float dxv = (float)(a.row_pos[b] - a.cpos) * a.dx;
       const float d2 = uniform_f(dxv * dxv);
       if (d2 > R2max) continue;
-      row_sweep(reinterpret_cast<const char*>(a.xp + (size_t)b * row_stride), nullptr, d2);
+      row_sweep(reinterpret_cast<const char*>(a.xp) + (size_t)b * row_stride * sizeof(float2), true,
+                zero, false, d2);
     }
   } else {
     const int c = tk.c;
@@ -178,8 +180,9 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
       const bool hp = rp >= tk.b0 && rp < tk.b1;
       const bool hm = d > 0 && rm >= tk.b0 && rm < tk.b1;
       if (!hp && !hm) continue;
-      row_sweep(hp ? reinterpret_cast<const char*>(a.xp + (size_t)rp * row_stride) : nullptr,
-                hm ? reinterpret_cast<const char*>(a.xp + (size_t)rm * row_stride) : nullptr, d2);
+      const char* base = reinterpret_cast<const char*>(a.xp);
+      row_sweep(hp ? base + (size_t)rp * row_stride * sizeof(float2) : zero, hp,
+                hm ? base + (size_t)rm * row_stride * sizeof(float2) : zero, hm, d2);
     }
   }
 }
@@ -256,6 +259,8 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void migrate_kernel(MigArgs
       sum_aperture<GENERIC_ROWS, false, COUNT>(a, tk, kj, p, R2max, acc, cap, cg);
   }
   if constexpr (COUNT) {
+    if (active && a.out) orow[s] = (float)cap;
+    if (active && a.out_g) a.out_g[((size_t)io * a.n_out_xl + j) * a.n_t + s] = (float)cg;
     const unsigned tap = wave_sum_u32(cap), tg = wave_sum_u32(cg);
     if (lane == 0) {
       atomicAdd(&a.counters[0], (unsigned long long)tap);

@@ -202,12 +202,29 @@ def migrate_volume(data, vel, params: SurveyParams, out=None, ctx: _lib.Context 
     return out
 
 
-def count_pairs(vel, params: SurveyParams, ctx: _lib.Context | None = None):
-    """(P_ap, P_g) of the work unit for this survey, counted on the GPU."""
+def count_pairs(vel, params: SurveyParams, ctx: _lib.Context | None = None, images=False):
+    """(P_ap, P_g) of the work unit for this survey, counted on the GPU.  With
+    images=True also returns the per-output-sample counts (float32 arrays of
+    the output shape, on the device when vel is a device tensor)."""
     ctx = ctx or _lib.context()
     vp, vw = _ptr_where(vel, "vel")
     s = params.to_c()
     pap, pg = C.c_double(), C.c_double()
-    ctx.check(ctx.lib.psm_count_pairs(ctx.h, C.byref(s), vp, vw, C.byref(pap), C.byref(pg)),
+    ap_img = g_img = None
+    if images:
+        info = survey_plan(params, vel)
+        shape = (info.n_out_il, info.n_out_xl, info.n_t)
+        if _is_torch(vel) and vw == PSM_DEVICE:
+            import torch
+            ap_img = torch.empty(shape, dtype=torch.float32, device=vel.device)
+            g_img = torch.empty(shape, dtype=torch.float32, device=vel.device)
+        else:
+            ap_img = np.empty(shape, np.float32)
+            g_img = np.empty(shape, np.float32)
+    ctx.check(ctx.lib.psm_count_pairs(ctx.h, C.byref(s), vp, vw, C.byref(pap), C.byref(pg),
+                                      None if ap_img is None else _ptr_where(ap_img, "ap")[0],
+                                      None if g_img is None else _ptr_where(g_img, "g")[0]),
               "psm_count_pairs")
+    if images:
+        return pap.value, pg.value, ap_img, g_img
     return pap.value, pg.value

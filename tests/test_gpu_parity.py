@@ -177,3 +177,26 @@ def test_count_pairs_matches_oracle(gpu_ctx):
     vel2 = synth.production_velocity(*n)
     prm2 = P.SurveyParams.for_cube(n, inline_padding=3, crossline_padding=4, angle_aperture=45.0)
     assert P.count_pairs(vel2, prm2) == O.count_pairs(vel2, 0.002, 25, 25, 3, 4, 45.0)
+
+
+def test_count_pairs_fresh_context_and_images():
+    """Counting on a context that has never migrated (no trace buffer yet),
+    with per-sample count images on host and device."""
+    import torch
+    from poststackmig3d_amd._lib import Context
+    n = (23, 19, 90)
+    vel = synth.production_velocity(*n)
+    prm = P.SurveyParams.for_cube(n, inline_padding=3, crossline_padding=4, angle_aperture=45.0)
+    want = O.count_pairs(vel, 0.002, 25, 25, 3, 4, 45.0)
+    ctx = Context(0)
+    try:
+        pap, pg, ap_img, g_img = P.count_pairs(vel, prm, ctx=ctx, images=True)
+        assert (pap, pg) == want
+        assert float(ap_img.astype(np.float64).sum()) == pap
+        assert float(g_img.astype(np.float64).sum()) == pg
+        assert (g_img <= ap_img).all() and not ap_img[:, :, 0].any()
+        pap2, pg2, ap_d, g_d = P.count_pairs(torch.from_numpy(vel).cuda(), prm, ctx=ctx, images=True)
+        assert (pap2, pg2) == want
+        assert np.array_equal(ap_d.cpu().numpy(), ap_img)
+    finally:
+        ctx.close()
