@@ -144,6 +144,15 @@ int psm_count_pairs(psm_ctx* ctx, const psm_survey* survey, const float* vel,
                     int where, double* p_ap, double* p_g,
                     float* ap_image, float* g_image);
 
+/* Launch shape of the migration kernel (no reference counterpart):
+ * waves_per_block = output crosslines per workgroup (1, 2, 4, 8 or 16);
+ * rows_per_pass = aperture row offsets summed per launch (0 = the whole
+ * aperture in one launch; smaller values bound each launch's cache working
+ * set to a few aperture rows, later launches accumulate).  Results do not
+ * depend on the shape beyond FP32 summation order. */
+int psm_set_tuning(psm_ctx* ctx, int waves_per_block, int rows_per_pass);
+int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
+
 /* Kernel-level timing (no reference counterpart): when enabled, each
  * psm_migrate_volume records HIP events on the context's stream around the
  * trace packing kernel and the migration kernel; psm_last_timing waits for

@@ -74,6 +74,8 @@ SIGNATURES = [
     ("psm_count_pairs", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p, C.c_int,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p,
                                   C.c_void_p]),
+    ("psm_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("psm_get_tuning", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("psm_enable_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("psm_last_timing", C.c_int, [C.c_void_p, C.POINTER(Timing)]),
 ]
@@ -134,6 +136,15 @@ class Context:
 
     def synchronize(self):
         self.check(self.lib.psm_synchronize(self.h), "psm_synchronize")
+
+    def set_tuning(self, waves_per_block: int = 4, rows_per_pass: int = 0):
+        self.check(self.lib.psm_set_tuning(self.h, int(waves_per_block), int(rows_per_pass)),
+                   "psm_set_tuning")
+
+    def get_tuning(self):
+        w, r = C.c_int(), C.c_int()
+        self.check(self.lib.psm_get_tuning(self.h, C.byref(w), C.byref(r)), "psm_get_tuning")
+        return w.value, r.value
 
     def enable_timing(self, on: bool = True):
         self.check(self.lib.psm_enable_timing(self.h, 1 if on else 0), "psm_enable_timing")

@@ -57,6 +57,7 @@ struct psm_ctx {
   bool timing = false;
   bool timed = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  psm::Tuning tune;
 };
 
 namespace {
@@ -326,11 +327,13 @@ int psm_kirchhoff_inline(psm_ctx* c, const float* input_buffer, size_t n_buf_il,
 
   const size_t n_traces = n_buf_il * n_buf_xl;
   const size_t in_bytes = n_traces * n_t * sizeof(float);
-  PSM_ALLOC(c, c->pairs, std::max<size_t>(n_traces * n_t * sizeof(float2), 8));
-  if (n_traces) {
-    PSM_HIP(c, upload(c->raw, input_buffer, in_bytes, st));
-    PSM_HIP(c, psm::launch_pack_pairs(c->raw.as<float>(), c->pairs.as<float2>(), n_traces, (int)n_t, st));
-  }
+  if ((n_buf_xl + 1) * n_t * sizeof(float2) >= (size_t)INT32_MAX)
+    return fail(c, PSM_ERR_ARG, "one inline of the buffer exceeds 2 GiB");
+  const size_t pair_elems = psm::pairs_elems((int)n_buf_il, (int)n_buf_xl, (int)n_t);
+  PSM_ALLOC(c, c->pairs, pair_elems * sizeof(float2));
+  if (n_traces) PSM_HIP(c, upload(c->raw, input_buffer, in_bytes, st));
+  PSM_HIP(c, psm::launch_pack_pairs(c->raw.as<float>(), c->pairs.as<float2>(), (int)n_buf_il,
+                                    (int)n_buf_xl, (int)n_t, st));
   PSM_HIP(c, upload(c->vel, velocity_slice, n_buf_xl * n_t * sizeof(float), st));
   PSM_HIP(c, upload(c->t0, t0_times, n_t * sizeof(float), st));
   psm::Task tk{contiguous ? (int)crow : 0, 0, (int)n_buf_il, 0};
@@ -342,6 +345,7 @@ int psm_kirchhoff_inline(psm_ctx* c, const float* input_buffer, size_t n_buf_il,
   psm::MigArgs a{};
   a.xp = c->pairs.as<float2>();
   a.zero_tr = c->zero.as<float2>();
+  a.zero_row = c->pairs.as<float2>() + (size_t)n_buf_il * (n_buf_xl + 1) * n_t;
   a.vel = c->vel.as<float>();
   a.out = c->out.as<float>();
   a.t0 = c->t0.as<float>();
@@ -368,7 +372,12 @@ int psm_kirchhoff_inline(psm_ctx* c, const float* input_buffer, size_t n_buf_il,
     // no aperture traces: every valid sample sums to zero
     a.tasks = c->tasks.as<psm::Task>();
   }
-  PSM_HIP(c, psm::launch_migrate(a, false, st));
+  int extent = 0;
+  if (a.row_pos)
+    extent = (int)n_buf_il - 1;
+  else if (n_buf_il)
+    extent = (int)std::max<long long>(0, std::max<long long>((long long)n_buf_il - 1 - crow, crow));
+  PSM_HIP(c, psm::launch_migrate(a, false, st, c->tune, extent));
   PSM_HIP(c, hipMemcpyAsync(output_slice, c->out.p, n_xl * n_t * sizeof(float), hipMemcpyDeviceToHost, st));
   PSM_HIP(c, hipStreamSynchronize(st));
   return PSM_OK;
@@ -413,9 +422,12 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
   const size_t K = s->n_xl + 2 * (size_t)padx;
 
   std::vector<psm::Task> tasks(plan.n_out_il);
-  for (size_t io = 0; io < plan.n_out_il; ++io)
+  int extent = 0;
+  for (size_t io = 0; io < plan.n_out_il; ++io) {
     tasks[io] = psm::Task{plan.out_to_in_il[io], plan.window[2 * io], plan.window[2 * io + 1],
                           plan.out_to_in_il[io]};
+    extent = std::max(extent, std::max(tasks[io].b1 - 1 - tasks[io].c, tasks[io].c - tasks[io].b0));
+  }
   PSM_HIP(c, upload(c->tasks, tasks.data(), tasks.size() * sizeof(psm::Task), st));
   PSM_HIP(c, upload(c->t0, plan.t0_times, nt * sizeof(float), st));
 
@@ -429,14 +441,19 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
       d_data = c->raw.as<float>();
     }
   }
-  if (!count) PSM_ALLOC(c, c->pairs, std::max<size_t>(cube * sizeof(float2), 8));
+  if (((size_t)s->n_xl + 1) * nt * sizeof(float2) >= (size_t)INT32_MAX)
+    return fail(c, PSM_ERR_ARG, "one inline of the survey exceeds 2 GiB");
+  const size_t pair_elems = psm::pairs_elems((int)s->n_il, (int)s->n_xl, (int)nt);
+  if (!count) PSM_ALLOC(c, c->pairs, pair_elems * sizeof(float2));
   c->timed = false;
   if (c->timing && !count) {
     for (hipEvent_t& e : c->ev)
       if (!e) PSM_HIP(c, hipEventCreate(&e));
     PSM_HIP(c, hipEventRecord(c->ev[0], st));
   }
-  if (!count) PSM_HIP(c, psm::launch_pack_pairs(d_data, c->pairs.as<float2>(), n_tr, (int)nt, st));
+  if (!count)
+    PSM_HIP(c, psm::launch_pack_pairs(d_data, c->pairs.as<float2>(), (int)s->n_il, (int)s->n_xl,
+                                      (int)nt, st));
   if (c->timing && !count) PSM_HIP(c, hipEventRecord(c->ev[1], st));
 
   float* d_out = out;
@@ -449,6 +466,7 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
   psm::MigArgs a{};
   a.xp = c->pairs.as<float2>();
   a.zero_tr = c->zero.as<float2>();
+  a.zero_row = count ? nullptr : c->pairs.as<float2>() + (size_t)s->n_il * (s->n_xl + 1) * nt;
   a.vel = d_vel;
   a.out = d_out;
   a.t0 = c->t0.as<float>();
@@ -487,7 +505,7 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
         a.out_g = g_image ? c->out.as<float>() + out_elems : nullptr;
       }
     }
-    PSM_HIP(c, psm::launch_migrate(a, true, st));
+    PSM_HIP(c, psm::launch_migrate(a, true, st, c->tune, extent));
     unsigned long long h[2] = {0, 0};
     PSM_HIP(c, hipMemcpyAsync(h, c->counters.p, sizeof h, hipMemcpyDeviceToHost, st));
     if (where == PSM_HOST) {
@@ -501,7 +519,7 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
     if (p_g) *p_g = (double)h[1];
     return PSM_OK;
   }
-  PSM_HIP(c, psm::launch_migrate(a, false, st));
+  PSM_HIP(c, psm::launch_migrate(a, false, st, c->tune, extent));
   if (c->timing) {
     PSM_HIP(c, hipEventRecord(c->ev[2], st));
     c->timed = true;
@@ -529,6 +547,26 @@ int psm_count_pairs(psm_ctx* c, const psm_survey* s, const float* vel, int where
   if (!c) return PSM_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
   return volume_impl(c, s, nullptr, vel, nullptr, where, true, p_ap, p_g, ap_image, g_image);
+}
+
+int psm_set_tuning(psm_ctx* c, int waves_per_block, int rows_per_pass) {
+  if (!c) return PSM_ERR_ARG;
+  if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4 &&
+      waves_per_block != 8 && waves_per_block != 16)
+    return fail(c, PSM_ERR_ARG, "waves_per_block must be 1, 2, 4, 8 or 16");
+  if (rows_per_pass < 0) return fail(c, PSM_ERR_ARG, "rows_per_pass must be >= 0");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->tune.waves_per_block = waves_per_block;
+  c->tune.rows_per_pass = rows_per_pass;
+  return PSM_OK;
+}
+
+int psm_get_tuning(psm_ctx* c, int* waves_per_block, int* rows_per_pass) {
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (waves_per_block) *waves_per_block = c->tune.waves_per_block;
+  if (rows_per_pass) *rows_per_pass = c->tune.rows_per_pass;
+  return PSM_OK;
 }
 
 int psm_enable_timing(psm_ctx* c, int on) {

@@ -20,9 +20,11 @@ struct Task {
 // [xd_off, xd_off + n_xd), the velocity crosslines [xv_off, xv_off + n_xv),
 // and K is the buffer width n_buf_xl that clamps the crossline window.
 struct MigArgs {
-  const float2* xp;        // traces in pair layout [n_rows][n_xd][n_t]: (x[i], x[i+1]-x[i]),
-                           // sample n_t-1 holds (0, 0)
-  const float2* zero_tr;   // one trace of n_t (0, 0) pairs, read for absent partners
+  const float2* xp;        // traces in pair layout [n_rows + 1][n_xd + 1][n_t]:
+                           // (x[i], x[i+1]-x[i]); sample n_t-1, the last trace of
+                           // each row and the last row hold (0, 0)
+  const float2* zero_tr;   // a trace of n_t (0, 0) pairs (slow paths' absent partners)
+  const float2* zero_row;  // the zero row of xp (fast path's absent rows)
   const float* vel;        // velocity rows [n_vrows][n_xv][n_t]
   float* out;              // [n_out_il][n_out_xl][n_t]
   const float* t0;         // [n_t]
@@ -35,12 +37,27 @@ struct MigArgs {
   int cpos;                // generic-row mode: position of the current inline
   float inv_dt, dx, dy, inv_dy, tan2;
   int amp;
+  int d_lo, d_hi;          // this pass: row offsets |db| in [d_lo, d_hi) (generic rows: row indices)
+  int accumulate;          // 0: out = sum; 1: out += sum (later passes)
+};
+
+// Launch-shape parameters (chosen by the autotuner or the caller).
+struct Tuning {
+  int waves_per_block = 4;   // output crosslines per workgroup (1, 2, 4, 8, 16)
+  int rows_per_pass = 0;     // aperture row offsets per launch; 0 = all in one launch
 };
 
 // Launchers (psm_kernels.hip).  All enqueue on `stream` and return the HIP
 // status of the launch.
-hipError_t launch_pack_pairs(const float* x, float2* xp, size_t n_traces, int n_t,
+// Elements (float2) of the pair layout of n_rows x n_xd traces of n_t samples.
+size_t pairs_elems(int n_rows, int n_xd, int n_t);
+// x: [n_rows][n_xd][n_t] raw traces -> xp: pair layout (pairs_elems entries).
+hipError_t launch_pack_pairs(const float* x, float2* xp, int n_rows, int n_xd, int n_t,
                              hipStream_t stream);
-hipError_t launch_migrate(const MigArgs& a, bool count_only, hipStream_t stream);
+// row_extent: largest row offset (generic rows: largest row index) any task
+// can reach; the launcher splits [0, row_extent] into passes of
+// tune.rows_per_pass row offsets.
+hipError_t launch_migrate(const MigArgs& a, bool count_only, hipStream_t stream,
+                          const Tuning& tune, int row_extent);
 
 }  // namespace psm

@@ -200,3 +200,34 @@ def test_count_pairs_fresh_context_and_images():
         assert np.array_equal(ap_d.cpu().numpy(), ap_img)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("wpb,rows", [(1, 0), (2, 3), (4, 1), (8, 2), (16, 5)])
+def test_volume_launch_shapes(wpb, rows):
+    """Every launch shape (workgroup width, aperture-row passes) gives the
+    reference's result."""
+    from poststackmig3d_amd._lib import Context
+    n = (29, 31, 140)
+    data, _ = synth.production(*n, n_diffractors=4, seed=8)
+    vel = synth.production_velocity(*n)
+    ref = reference_volume(data, vel, 0.002, 25.0, 25.0, 3, 4, 45.0, True)
+    prm = P.SurveyParams.for_cube(n, inline_padding=3, crossline_padding=4, angle_aperture=45.0)
+    ctx = Context(0)
+    try:
+        ctx.set_tuning(wpb, rows)
+        assert ctx.get_tuning() == (wpb, rows)
+        got = P.migrate_volume(data, vel, prm, ctx=ctx)
+        assert rel_l2(got, ref) <= TOL
+        # the per-inline drop-in under the same shape
+        vmax = O.estimate_max_velocity(vel)
+        tan2, ap = O.aperture(45.0, vmax, n[2], 0.002, 25.0)
+        buf, vs, il_pos, cur = window_call(data, vel, n[0] - 2, ap, 3, 4)
+        t0 = t0_table(n[2], 0.002)
+        r1 = reference_inline(buf, vs, il_pos, cur, 0, 0.002, t0, tan2, 25.0, 25.0, True,
+                              buf.shape[1])
+        out = np.empty_like(r1)
+        P.kirchhoff_kernel_3d_with_padding(buf, *buf.shape, vs, out, il_pos, None, cur, 0, 0.002,
+                                           t0, tan2, 25.0, 25.0, True, ctx=ctx)
+        assert rel_l2(out, r1) <= TOL
+    finally:
+        ctx.close()
