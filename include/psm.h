@@ -43,6 +43,12 @@ extern "C" {
 #define PSM_HOST 0        /* pointer arguments are host memory */
 #define PSM_DEVICE 1      /* pointer arguments are device memory (HBM) */
 
+/* Storage of the input traces in HBM (psm_set_storage).  Travel time,
+ * interpolation weights and accumulation are float32 in every form. */
+#define PSM_STORE_F32 0   /* float32 samples (the reference's arithmetic) */
+#define PSM_STORE_F16 1   /* float16, each trace scaled by 1/max|x| */
+#define PSM_STORE_BF16 2  /* bfloat16, each trace scaled by 1/max|x| */
+
 typedef struct psm_ctx psm_ctx;
 
 /* Library / context lifetime. */
@@ -152,6 +158,12 @@ int psm_count_pairs(psm_ctx* ctx, const psm_survey* survey, const float* vel,
  * depend on the shape beyond FP32 summation order. */
 int psm_set_tuning(psm_ctx* ctx, int waves_per_block, int rows_per_pass);
 int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
+
+/* Trace storage used by later calls on this context (no reference
+ * counterpart): PSM_STORE_F32 (default), PSM_STORE_F16 or PSM_STORE_BF16.
+ * The reduced forms halve the bytes each gather reads; results then differ
+ * from the float32 form by the rounding of the stored samples. */
+int psm_set_storage(psm_ctx* ctx, int storage);
 
 /* Kernel-level timing (no reference counterpart): when enabled, each
  * psm_migrate_volume records HIP events on the context's stream around the

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libpsm_hip.so")
 PSM_OK = 0
 PSM_HOST = 0
 PSM_DEVICE = 1
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
@@ -74,6 +75,7 @@ SIGNATURES = [
     ("psm_count_pairs", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p, C.c_int,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p,
                                   C.c_void_p]),
+    ("psm_set_storage", C.c_int, [C.c_void_p, C.c_int]),
     ("psm_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("psm_get_tuning", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("psm_enable_timing", C.c_int, [C.c_void_p, C.c_int]),
@@ -136,6 +138,12 @@ class Context:
 
     def synchronize(self):
         self.check(self.lib.psm_synchronize(self.h), "psm_synchronize")
+
+    def set_storage(self, storage: str = "f32"):
+        """Trace storage in HBM: 'f32', 'f16' or 'bf16' (per-trace scaled)."""
+        if storage not in STORAGE:
+            raise PsmError(f"unknown storage {storage!r}")
+        self.check(self.lib.psm_set_storage(self.h, STORAGE[storage]), "psm_set_storage")
 
     def set_tuning(self, waves_per_block: int = 4, rows_per_pass: int = 0):
         self.check(self.lib.psm_set_tuning(self.h, int(waves_per_block), int(rows_per_pass)),

@@ -51,8 +51,8 @@ struct psm_ctx {
   hipStream_t stream = nullptr;
   std::mutex mu;
   std::string err;
-  DevBuf raw, pairs, vel, out, t0, tasks, rowpos, counters, zero;
-  size_t zero_nt = 0;
+  DevBuf raw, pairs, scale, vel, out, t0, tasks, rowpos, counters;
+  int storage = PSM_STORE_F32;
   // kernel-level timing of the last volume call (psm_enable_timing)
   bool timing = false;
   bool timed = false;
@@ -223,16 +223,6 @@ int plan_impl(psm_ctx* c, const psm_survey* s, const float* vel_probe, int where
   return PSM_OK;
 }
 
-// A trace of n_t zero pairs, read by the kernel in place of absent traces.
-hipError_t ensure_zero_trace(psm_ctx* c, size_t n_t, hipStream_t st) {
-  if (c->zero_nt >= n_t && c->zero.p) return hipSuccess;
-  hipError_t e = c->zero.ensure(n_t * sizeof(float2));
-  if (e != hipSuccess) return e;
-  e = hipMemsetAsync(c->zero.p, 0, n_t * sizeof(float2), st);
-  if (e == hipSuccess) c->zero_nt = n_t;
-  return e;
-}
-
 hipError_t upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
   hipError_t e = b.ensure(bytes);
   if (e != hipSuccess) return e;
@@ -267,7 +257,7 @@ void psm_destroy(psm_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (DevBuf* b : {&c->raw, &c->pairs, &c->vel, &c->out, &c->t0, &c->tasks, &c->rowpos, &c->counters, &c->zero})
+  for (DevBuf* b : {&c->raw, &c->pairs, &c->scale, &c->vel, &c->out, &c->t0, &c->tasks, &c->rowpos, &c->counters})
     b->release();
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -329,11 +319,11 @@ int psm_kirchhoff_inline(psm_ctx* c, const float* input_buffer, size_t n_buf_il,
   const size_t in_bytes = n_traces * n_t * sizeof(float);
   if ((n_buf_xl + 1) * n_t * sizeof(float2) >= (size_t)INT32_MAX)
     return fail(c, PSM_ERR_ARG, "one inline of the buffer exceeds 2 GiB");
-  const size_t pair_elems = psm::pairs_elems((int)n_buf_il, (int)n_buf_xl, (int)n_t);
-  PSM_ALLOC(c, c->pairs, pair_elems * sizeof(float2));
+  PSM_ALLOC(c, c->pairs, psm::pairs_bytes(c->storage, (int)n_buf_il, (int)n_buf_xl, (int)n_t));
+  PSM_ALLOC(c, c->scale, (n_buf_il + 1) * (n_buf_xl + 1) * sizeof(float));
   if (n_traces) PSM_HIP(c, upload(c->raw, input_buffer, in_bytes, st));
-  PSM_HIP(c, psm::launch_pack_pairs(c->raw.as<float>(), c->pairs.as<float2>(), (int)n_buf_il,
-                                    (int)n_buf_xl, (int)n_t, st));
+  PSM_HIP(c, psm::launch_pack(c->storage, c->raw.as<float>(), c->pairs.p, c->scale.as<float>(),
+                              (int)n_buf_il, (int)n_buf_xl, (int)n_t, st));
   PSM_HIP(c, upload(c->vel, velocity_slice, n_buf_xl * n_t * sizeof(float), st));
   PSM_HIP(c, upload(c->t0, t0_times, n_t * sizeof(float), st));
   psm::Task tk{contiguous ? (int)crow : 0, 0, (int)n_buf_il, 0};
@@ -341,11 +331,11 @@ int psm_kirchhoff_inline(psm_ctx* c, const float* input_buffer, size_t n_buf_il,
   if (!contiguous && n_buf_il) PSM_HIP(c, upload(c->rowpos, il_pos, n_buf_il * sizeof(int32_t), st));
   PSM_ALLOC(c, c->out, n_xl * n_t * sizeof(float));
 
-  PSM_HIP(c, ensure_zero_trace(c, n_t, st));
   psm::MigArgs a{};
-  a.xp = c->pairs.as<float2>();
-  a.zero_tr = c->zero.as<float2>();
-  a.zero_row = c->pairs.as<float2>() + (size_t)n_buf_il * (n_buf_xl + 1) * n_t;
+  a.xp = c->pairs.p;
+  a.scale = c->scale.as<float>();
+  a.n_rows = (int)n_buf_il;
+  a.storage = c->storage;
   a.vel = c->vel.as<float>();
   a.out = c->out.as<float>();
   a.t0 = c->t0.as<float>();
@@ -443,8 +433,10 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
   }
   if (((size_t)s->n_xl + 1) * nt * sizeof(float2) >= (size_t)INT32_MAX)
     return fail(c, PSM_ERR_ARG, "one inline of the survey exceeds 2 GiB");
-  const size_t pair_elems = psm::pairs_elems((int)s->n_il, (int)s->n_xl, (int)nt);
-  if (!count) PSM_ALLOC(c, c->pairs, pair_elems * sizeof(float2));
+  if (!count) {
+    PSM_ALLOC(c, c->pairs, psm::pairs_bytes(c->storage, (int)s->n_il, (int)s->n_xl, (int)nt));
+    PSM_ALLOC(c, c->scale, (s->n_il + 1) * (s->n_xl + 1) * sizeof(float));
+  }
   c->timed = false;
   if (c->timing && !count) {
     for (hipEvent_t& e : c->ev)
@@ -452,8 +444,8 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
     PSM_HIP(c, hipEventRecord(c->ev[0], st));
   }
   if (!count)
-    PSM_HIP(c, psm::launch_pack_pairs(d_data, c->pairs.as<float2>(), (int)s->n_il, (int)s->n_xl,
-                                      (int)nt, st));
+    PSM_HIP(c, psm::launch_pack(c->storage, d_data, c->pairs.p, c->scale.as<float>(), (int)s->n_il,
+                                (int)s->n_xl, (int)nt, st));
   if (c->timing && !count) PSM_HIP(c, hipEventRecord(c->ev[1], st));
 
   float* d_out = out;
@@ -462,11 +454,11 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
     PSM_ALLOC(c, c->out, out_elems * sizeof(float));
     d_out = c->out.as<float>();
   }
-  PSM_HIP(c, ensure_zero_trace(c, nt, st));
   psm::MigArgs a{};
-  a.xp = c->pairs.as<float2>();
-  a.zero_tr = c->zero.as<float2>();
-  a.zero_row = count ? nullptr : c->pairs.as<float2>() + (size_t)s->n_il * (s->n_xl + 1) * nt;
+  a.xp = c->pairs.p;
+  a.scale = c->scale.as<float>();
+  a.n_rows = (int)s->n_il;
+  a.storage = c->storage;
   a.vel = d_vel;
   a.out = d_out;
   a.t0 = c->t0.as<float>();
@@ -547,6 +539,15 @@ int psm_count_pairs(psm_ctx* c, const psm_survey* s, const float* vel, int where
   if (!c) return PSM_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
   return volume_impl(c, s, nullptr, vel, nullptr, where, true, p_ap, p_g, ap_image, g_image);
+}
+
+int psm_set_storage(psm_ctx* c, int storage) {
+  if (!c) return PSM_ERR_ARG;
+  if (storage != PSM_STORE_F32 && storage != PSM_STORE_F16 && storage != PSM_STORE_BF16)
+    return fail(c, PSM_ERR_ARG, "unknown storage %d", storage);
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->storage = storage;
+  return PSM_OK;
 }
 
 int psm_set_tuning(psm_ctx* c, int waves_per_block, int rows_per_pass) {

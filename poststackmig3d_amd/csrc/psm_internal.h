@@ -7,6 +7,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "psm.h"
+
 namespace psm {
 
 // One output inline of work: the centre row (in data-row index space), the
@@ -20,11 +22,12 @@ struct Task {
 // [xd_off, xd_off + n_xd), the velocity crosslines [xv_off, xv_off + n_xv),
 // and K is the buffer width n_buf_xl that clamps the crossline window.
 struct MigArgs {
-  const float2* xp;        // traces in pair layout [n_rows + 1][n_xd + 1][n_t]:
-                           // (x[i], x[i+1]-x[i]); sample n_t-1, the last trace of
-                           // each row and the last row hold (0, 0)
-  const float2* zero_tr;   // a trace of n_t (0, 0) pairs (slow paths' absent partners)
-  const float2* zero_row;  // the zero row of xp (fast path's absent rows)
+  const void* xp;          // traces in pair layout [n_rows + 1][n_xd + 1][n_t] of
+                           // (x[i], x[i+1]-x[i]) in the `storage` format; sample n_t-1,
+                           // the last trace of each row and the last row are zero
+  const float* scale;      // [n_rows + 1][n_xd + 1] per-trace scale (F16 / BF16 storage)
+  int n_rows;              // data rows (inlines) in xp
+  int storage;             // PSM_STORE_*
   const float* vel;        // velocity rows [n_vrows][n_xv][n_t]
   float* out;              // [n_out_il][n_out_xl][n_t]
   const float* t0;         // [n_t]
@@ -49,11 +52,12 @@ struct Tuning {
 
 // Launchers (psm_kernels.hip).  All enqueue on `stream` and return the HIP
 // status of the launch.
-// Elements (float2) of the pair layout of n_rows x n_xd traces of n_t samples.
-size_t pairs_elems(int n_rows, int n_xd, int n_t);
-// x: [n_rows][n_xd][n_t] raw traces -> xp: pair layout (pairs_elems entries).
-hipError_t launch_pack_pairs(const float* x, float2* xp, int n_rows, int n_xd, int n_t,
-                             hipStream_t stream);
+// Bytes of the pair layout of n_rows x n_xd traces of n_t samples.
+size_t pairs_bytes(int storage, int n_rows, int n_xd, int n_t);
+// x: [n_rows][n_xd][n_t] raw traces -> xp: pair layout in `storage` format,
+// scale: [n_rows + 1][n_xd + 1] per-trace scales (written for F16 / BF16).
+hipError_t launch_pack(int storage, const float* x, void* xp, float* scale, int n_rows, int n_xd,
+                       int n_t, hipStream_t stream);
 // row_extent: largest row offset (generic rows: largest row index) any task
 // can reach; the launcher splits [0, row_extent] into passes of
 // tune.rows_per_pass row offsets.

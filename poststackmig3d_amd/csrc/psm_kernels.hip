@@ -13,20 +13,27 @@
 //
 // Mapping to CDNA4:
 //   * one wavefront = one output trace x 64 consecutive t0 samples (lane =
-//     sample), so the aperture loop is wave-uniform and r^2 lives in scalar
-//     registers; the 64 gathers of one aperture trace touch one contiguous
+//     sample), so the aperture loop is wave-uniform and all trace addressing
+//     is scalar; the 64 gathers of one aperture trace touch one contiguous
 //     stretch of that trace (dtau/dt0 <= 1);
 //   * the aperture is walked in (|db|, |dk|) order and each (|db|, |dk|)
 //     shares tau, i, f and the weight between the up-to-four traces
 //     (c +- db, kj +- dk) that have the same r^2 -- the travel time depends
 //     only on r^2 and on the image point's velocity;
 //   * traces are stored in a pair layout (x[i], x[i+1]-x[i]) so that one
-//     8-byte load and one FMA give the linear interpolation of
+//     gather and one FMA give the linear interpolation of
 //     migration_kernel.cpp:544-550, and the last sample holds (0,0) so the
-//     bound i < n_t-1 (:298) becomes a clamp instead of a mask;
+//     bound i < n_t-1 (:298) becomes a clamp instead of a mask.  The pair is
+//     two float32 (8 bytes) or, for the reduced-precision storage forms, two
+//     float16 / bfloat16 of the trace divided by its max |x| (4 bytes), with
+//     the per-trace scale applied in float32;
+//   * the gathers are buffer loads with a scalar trace offset, two steps in
+//     flight per wave; they are bound by the texture-address issue rate,
+//     which is why halving the pair width matters;
 //   * workgroups are ordered latest t0 tile first (work grows as t0^2) and
 //     striped so that each XCD takes a contiguous slab of output traces of
-//     every tile (its own L2 sees neighbouring apertures).
+//     every tile; long apertures may be split into launches over a few
+//     aperture rows each so the in-flight working set stays in L2.
 //
 // Arithmetic follows the reference's compiled order: r^2 = fma(dk dy, dk dy,
 // dx^2) and tau^2 = fma(4/v^2, r^2, t0^2) are single-rounding FMAs (the
@@ -43,6 +50,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kXcds = 8;
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 buffer descriptor: 32-bit data format
 
 // Correctly rounded square root for positive normal x: v_sqrt_f32 is within
 // one ulp, and the neighbour whose square brackets x is chosen by the sign of
@@ -76,6 +84,59 @@ __device__ __forceinline__ float uniform_f(float v) {
 }
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Read-only data indexed by wave-uniform values is read through the constant
+// address space so that it goes to the scalar cache (s_load) instead of
+// occupying the vector memory path.
+using const_f32_ptr = const __attribute__((address_space(4))) float*;
+__device__ __forceinline__ const_f32_ptr as_const(const float* p) { return (const_f32_ptr)p; }
+
+// ---------------------------------------------------------------- storage
+// One stored pair (x[i], x[i+1] - x[i]) and its linear interpolation in fp32.
+template <int ST>
+struct Store;
+
+template <>
+struct Store<PSM_STORE_F32> {
+  using Raw = float2;
+  static constexpr int kBytes = 8;
+  static constexpr bool kScaled = false;
+  __device__ static Raw load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  }
+  __device__ static Raw load_ptr(const char* p) { return *reinterpret_cast<const float2*>(p); }
+  __device__ static float lerp(Raw x, float f) { return __builtin_fmaf(f, x.y, x.x); }
+};
+
+template <>
+struct Store<PSM_STORE_F16> {
+  using Raw = unsigned;
+  static constexpr int kBytes = 4;
+  static constexpr bool kScaled = true;
+  __device__ static Raw load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+  }
+  __device__ static Raw load_ptr(const char* p) { return *reinterpret_cast<const unsigned*>(p); }
+  __device__ static float lerp(Raw x, float f) {
+    const _Float16 lo = __builtin_bit_cast(_Float16, (unsigned short)(x & 0xffffu));
+    const _Float16 d = __builtin_bit_cast(_Float16, (unsigned short)(x >> 16));
+    return __builtin_fmaf(f, (float)d, (float)lo);
+  }
+};
+
+template <>
+struct Store<PSM_STORE_BF16> {
+  using Raw = unsigned;
+  static constexpr int kBytes = 4;
+  static constexpr bool kScaled = true;
+  __device__ static Raw load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+  }
+  __device__ static Raw load_ptr(const char* p) { return *reinterpret_cast<const unsigned*>(p); }
+  __device__ static float lerp(Raw x, float f) {
+    return __builtin_fmaf(f, __uint_as_float(x & 0xffff0000u), __uint_as_float(x << 16));
+  }
+};
+
 // Per-lane image-point parameters (migration_kernel.cpp:115-144), evaluated
 // for the crossline `src` whose velocity the lane uses.
 struct LaneParams {
@@ -83,13 +144,15 @@ struct LaneParams {
   int wlo, whi;
 };
 
-// One (|db|, |dk|) step of the aperture sum: the travel time, sample index,
-// fraction and weight shared by the up-to-four traces with the same r^2, and
-// their four pair-layout gathers.
+// One (|db|, |dk|) step of the aperture sum: the fraction, weight and mask
+// shared by the up-to-four traces with the same r^2, their four gathers and
+// (reduced-precision storage) their four trace scales.
+template <int ST>
 struct Step {
   float f, w;
   bool m;
-  float2 x0, x1, x2, x3;
+  typename Store<ST>::Raw x0, x1, x2, x3;
+  float s0, s1, s2, s3;
 };
 
 // Sums the aperture of one (output trace, t0 tile) over the row offsets
@@ -97,28 +160,31 @@ struct Step {
 // an earlier crossline whose crossline window can cut the aperture (sticky
 // samples of migration_kernel.cpp:64-152), so each gather is also checked
 // against that window.  COUNT = count pairs instead of summing.  All loop
-// bounds and trace pointers are wave-uniform (scalar registers); absent
-// partners read a trace of zeros so that the four gathers of a step are
-// issued back to back, and the fast path loads step dk+1 before it consumes
-// step dk (software pipelining: two steps of gathers in flight per wave).
-template <bool GENERIC_ROWS, bool WINDOW, bool COUNT>
+// bounds and trace offsets are wave-uniform (scalar registers); absent
+// crosslines read the row's zero trace and absent rows the zero row, so the
+// four gathers of a step are issued back to back, and the fast path loads
+// step dk+1 before it consumes step dk (two steps of gathers in flight).
+template <int ST, bool GENERIC_ROWS, bool WINDOW, bool COUNT>
 __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, int kj,
                                              const LaneParams& p, float R2max,
                                              float& acc, unsigned& cap, unsigned& cg) {
+  using S = Store<ST>;
   const int n_t = a.n_t;
   const int nt1 = n_t - 1;
   const int xd_lo = a.xd_off, xd_hi = a.xd_off + a.n_xd;
   // pair layout rows are [n_xd + 1][n_t]: the last trace of every row is zero
-  const size_t tr_bytes = (size_t)n_t * sizeof(float2);
-  const size_t row_bytes = (size_t)(a.n_xd + 1) * tr_bytes;
-  const char* const zero = reinterpret_cast<const char*>(a.zero_tr);
+  const int tb = n_t * S::kBytes;
+  const size_t row_bytes = (size_t)(a.n_xd + 1) * tb;
   const char* const base = reinterpret_cast<const char*>(a.xp);
-  const char* const zero_row = reinterpret_cast<const char*>(a.zero_row);
+  const char* const zero_row = base + (size_t)a.n_rows * row_bytes;
+  const int row_tr = a.n_xd + 1;  // traces per row (scales)
   // largest crossline offset that can hold data
   const int dk_lim = max(kj - xd_lo, xd_hi - 1 - kj);
 
-  // rowp / rowm: the rows at +db and -db (present when hp / hm)
-  auto row_sweep = [&](const char* rowp, bool hp, const char* rowm, bool hm, float d2) {
+  // rowp / rowm: rows at +db and -db (present when hp / hm), with their
+  // index for the scale table
+  auto row_sweep = [&](const char* rowp, bool hp, int ip, const char* rowm, bool hm, int im,
+                       float d2) {
     // last dk with r^2 <= R2max (r^2 is nondecreasing in dk)
     int kmax = (int)(__builtin_sqrtf(R2max - d2) * a.inv_dy) + 2;
     kmax = min(kmax, dk_lim);
@@ -128,60 +194,69 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
       --kmax;
     }
     kmax = uniform_i(kmax);
-
-    auto r2_of = [&](int dk) {
-      const float dyv = (float)dk * a.dy;
-      return uniform_f(__builtin_fmaf(dyv, dyv, d2));
-    };
+    if (kmax < 0) return;
+    const const_f32_ptr scp = as_const(a.scale + (size_t)ip * row_tr);
+    const const_f32_ptr scm = as_const(a.scale + (size_t)im * row_tr);
+    const int zero_off = a.n_xd * tb;  // the row's zero trace
+    const unsigned nxd = (unsigned)a.n_xd;
 
     if constexpr (!WINDOW && !COUNT) {
       // Fast path.  Each row is read through a buffer resource whose base is
       // the row; a gather's address is (scalar trace offset) + (lane sample
-      // offset), so a step costs a few scalar adds and no 64-bit address
-      // arithmetic.  Absent crosslines read the row's trailing zero trace,
-      // absent rows the trailing zero row.
-      const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
-          const_cast<char*>(rowp), 0, (int)row_bytes, 0x00020000);
-      const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc(
-          const_cast<char*>(rowm), 0, (int)row_bytes, 0x00020000);
-      const int tb = (int)tr_bytes;
-      const int zero_off = a.n_xd * tb;                 // the row's zero trace
-      const unsigned nxd = (unsigned)a.n_xd;
-      auto prep = [&](int dk, Step& g) {
+      // offset).
+      const __amdgpu_buffer_rsrc_t rp =
+          __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(rowp), 0, (int)row_bytes, kRsrcWord3);
+      const __amdgpu_buffer_rsrc_t rm =
+          __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(rowm), 0, (int)row_bytes, kRsrcWord3);
+      auto prep = [&](int dk, Step<ST>& g) {
         const float dyv = (float)dk * a.dy;
         const float r2 = __builtin_fmaf(dyv, dyv, d2);
-        const int xp_ = kj + dk - xd_lo, xm_ = kj - dk - xd_lo;   // data crossline indices
-        const int op = ((unsigned)xp_ < nxd) ? xp_ * tb : zero_off;
-        const int om = (dk > 0 && (unsigned)xm_ < nxd) ? xm_ * tb : zero_off;
+        const int xp_ = kj + dk - xd_lo, xm_ = kj - dk - xd_lo;  // data crossline indices
+        const bool vp = (unsigned)xp_ < nxd;
+        const bool vm = dk > 0 && (unsigned)xm_ < nxd;
+        const int op = vp ? xp_ * tb : zero_off;
+        const int om = vm ? xm_ * tb : zero_off;
         const float tau = sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq));
         const float q = tau * a.inv_dt;
-        const int voff = min((int)q, nt1) * (int)sizeof(float2);
-        g.x0 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rp, voff, op, 0));
-        g.x1 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rp, voff, om, 0));
-        g.x2 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rm, voff, op, 0));
-        g.x3 = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rm, voff, om, 0));
+        const int voff = min((int)q, nt1) * S::kBytes;
+        g.x0 = S::load(rp, voff, op);
+        g.x1 = S::load(rp, voff, om);
+        g.x2 = S::load(rm, voff, op);
+        g.x3 = S::load(rm, voff, om);
+        if constexpr (S::kScaled) {
+          const int sp = vp ? xp_ : a.n_xd, sm = vm ? xm_ : a.n_xd;
+          g.s0 = scp[sp];
+          g.s1 = scp[sm];
+          g.s2 = scm[sp];
+          g.s3 = scm[sm];
+        }
         g.f = __builtin_amdgcn_fractf(q);
         g.w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
         g.m = r2 <= p.R2;
       };
-      // a masked step adds 0 * w: the gathered sum is selected away before the
-      // multiply, so a non-finite sample outside the aperture cannot reach acc
-      auto consume = [&](const Step& g) {
-        const float l0 = __builtin_fmaf(g.f, g.x0.y, g.x0.x);
-        const float l1 = __builtin_fmaf(g.f, g.x1.y, g.x1.x);
-        const float l2 = __builtin_fmaf(g.f, g.x2.y, g.x2.x);
-        const float l3 = __builtin_fmaf(g.f, g.x3.y, g.x3.x);
-        const float sum = g.m ? (l0 + l1) + (l2 + l3) : 0.0f;
+      // a masked step adds 0 * w: the gathered sum is selected away before
+      // the multiply, so a non-finite sample outside the aperture cannot
+      // reach acc
+      auto consume = [&](const Step<ST>& g) {
+        const float l0 = S::lerp(g.x0, g.f);
+        const float l1 = S::lerp(g.x1, g.f);
+        const float l2 = S::lerp(g.x2, g.f);
+        const float l3 = S::lerp(g.x3, g.f);
+        float sum;
+        if constexpr (S::kScaled)
+          sum = __builtin_fmaf(l3, g.s3, __builtin_fmaf(l2, g.s2, __builtin_fmaf(l1, g.s1, l0 * g.s0)));
+        else
+          sum = (l0 + l1) + (l2 + l3);
+        sum = g.m ? sum : 0.0f;
         acc = __builtin_fmaf(sum, g.w, acc);
       };
-      if (kmax < 0) return;
       // Two steps in flight: the gathers of step dk+1 are issued before step
       // dk is consumed (the scheduling barriers keep the compiler from
       // hoisting the consumption above them).  Steps past kmax are prefetched
       // and consumed unconditionally, so the load counter is the same on
       // every path; such a step either has r^2 > R2max (mask false in every
       // lane) or lies beyond the data on both sides (reads the zero trace).
-      Step A, B;
+      Step<ST> A, B;
       prep(0, A);
       for (int dk = 0; dk <= kmax; dk += 2) {
         prep(dk + 1, B);
@@ -193,7 +268,8 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
       }
     } else {
       for (int dk = 0; dk <= kmax; ++dk) {
-        const float r2 = r2_of(dk);
+        const float dyv = (float)dk * a.dy;
+        const float r2 = __builtin_fmaf(dyv, dyv, d2);
         const int kp = kj + dk, km = kj - dk;
         const bool has_kp = kp >= xd_lo && kp < xd_hi;
         const bool has_km = dk > 0 && km >= xd_lo && km < xd_hi;
@@ -213,27 +289,28 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
           cap += in_mask ? n : 0u;
           cg += (in_mask && i < nt1) ? n : 0u;
         } else {
-          const char* t0p = (hp && has_kp) ? rowp + (size_t)(kp - xd_lo) * tr_bytes : zero;
-          const char* t0m = (hp && has_km) ? rowp + (size_t)(km - xd_lo) * tr_bytes : zero;
-          const char* t1p = (hm && has_kp) ? rowm + (size_t)(kp - xd_lo) * tr_bytes : zero;
-          const char* t1m = (hm && has_km) ? rowm + (size_t)(km - xd_lo) * tr_bytes : zero;
-          const unsigned off = (unsigned)min(i, nt1) * (unsigned)sizeof(float2);
-          const float2 x0 = *reinterpret_cast<const float2*>(t0p + off);
-          const float2 x1 = *reinterpret_cast<const float2*>(t0m + off);
-          const float2 x2 = *reinterpret_cast<const float2*>(t1p + off);
-          const float2 x3 = *reinterpret_cast<const float2*>(t1m + off);
+          const int xp_ = has_kp ? kp - xd_lo : a.n_xd;
+          const int xm_ = has_km ? km - xd_lo : a.n_xd;
+          const size_t off = (size_t)min(i, nt1) * S::kBytes;
           const float f = __builtin_amdgcn_fractf(q);
-          float w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
-          w = in_mask ? w : 0.0f;
-          float l0 = __builtin_fmaf(f, x0.y, x0.x);
-          float l1 = __builtin_fmaf(f, x1.y, x1.x);
-          float l2 = __builtin_fmaf(f, x2.y, x2.x);
-          float l3 = __builtin_fmaf(f, x3.y, x3.x);
+          float l0 = S::lerp(S::load_ptr(rowp + (size_t)xp_ * tb + off), f);
+          float l1 = S::lerp(S::load_ptr(rowp + (size_t)xm_ * tb + off), f);
+          float l2 = S::lerp(S::load_ptr(rowm + (size_t)xp_ * tb + off), f);
+          float l3 = S::lerp(S::load_ptr(rowm + (size_t)xm_ * tb + off), f);
+          if constexpr (S::kScaled) {
+            l0 *= scp[xp_];
+            l1 *= scp[xm_];
+            l2 *= scm[xp_];
+            l3 *= scm[xm_];
+          }
           l0 = okp ? l0 : 0.0f;
           l2 = okp ? l2 : 0.0f;
           l1 = okm ? l1 : 0.0f;
           l3 = okm ? l3 : 0.0f;
-          acc = __builtin_fmaf((l0 + l1) + (l2 + l3), w, acc);
+          float sum = (l0 + l1) + (l2 + l3);
+          sum = in_mask ? sum : 0.0f;
+          const float w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+          acc = __builtin_fmaf(sum, w, acc);
         }
       }
     }
@@ -246,7 +323,7 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
       const float dxv = (float)(a.row_pos[b] - a.cpos) * a.dx;
       const float d2 = uniform_f(dxv * dxv);
       if (d2 > R2max) continue;
-      row_sweep(base + (size_t)b * row_bytes, true, zero_row, false, d2);
+      row_sweep(base + (size_t)b * row_bytes, true, b, zero_row, false, a.n_rows, d2);
     }
   } else {
     const int c = tk.c;
@@ -261,13 +338,13 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
       const bool hp = rp >= tk.b0 && rp < tk.b1;
       const bool hm = d > 0 && rm >= tk.b0 && rm < tk.b1;
       if (!hp && !hm) continue;
-      row_sweep(hp ? base + (size_t)rp * row_bytes : zero_row, hp,
-                hm ? base + (size_t)rm * row_bytes : zero_row, hm, d2);
+      row_sweep(hp ? base + (size_t)rp * row_bytes : zero_row, hp, hp ? rp : a.n_rows,
+                hm ? base + (size_t)rm * row_bytes : zero_row, hm, hm ? rm : a.n_rows, d2);
     }
   }
 }
 
-template <int WPB, bool GENERIC_ROWS, bool COUNT>
+template <int ST, int WPB, bool GENERIC_ROWS, bool COUNT>
 __global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_tiles,
                                                               int n_jgroups, int per_tile_pad) {
   // ---- block -> (t0 tile, output inline, crossline group), heavy tiles first
@@ -332,9 +409,9 @@ __global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_t
   unsigned cap = 0, cg = 0;
   if (R2max >= 0.0f) {
     if (__any(needs_window))
-      sum_aperture<GENERIC_ROWS, true, COUNT>(a, tk, kj, p, R2max, acc, cap, cg);
+      sum_aperture<ST, GENERIC_ROWS, true, COUNT>(a, tk, kj, p, R2max, acc, cap, cg);
     else
-      sum_aperture<GENERIC_ROWS, false, COUNT>(a, tk, kj, p, R2max, acc, cap, cg);
+      sum_aperture<ST, GENERIC_ROWS, false, COUNT>(a, tk, kj, p, R2max, acc, cap, cg);
   }
   if constexpr (COUNT) {
     if (active && a.out) orow[s] = (float)cap;
@@ -353,47 +430,89 @@ __global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_t
   }
 }
 
-// Pair layout [n_rows + 1][n_xd + 1][n_t] of float2 (x[i], x[i+1] - x[i]):
-// sample n_t-1 of every trace, the last trace of every row and the whole last
-// row hold (0, 0).
-__global__ void pack_pairs_kernel(const float* __restrict__ x, float2* __restrict__ xp,
-                                  int n_rows, int n_xd, int n_t) {
-  const size_t n = (size_t)(n_rows + 1) * (n_xd + 1) * n_t;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += stride) {
-    const int i = (int)(idx % (size_t)n_t);
-    const size_t tr = idx / (size_t)n_t;
-    const int xl = (int)(tr % (size_t)(n_xd + 1));
-    const int row = (int)(tr / (size_t)(n_xd + 1));
-    float2 pr = make_float2(0.0f, 0.0f);
-    if (row < n_rows && xl < n_xd && i < n_t - 1) {
-      const size_t src = ((size_t)row * n_xd + xl) * n_t + i;
-      const float lo = x[src];
-      pr = make_float2(lo, x[src + 1] - lo);
+// ---------------------------------------------------------------- packing
+// Pair layout [n_rows + 1][n_xd + 1][n_t]: sample n_t-1 of every trace, the
+// last trace of every row and the whole last row are zero.  F32 stores
+// float2 (x[i], x[i+1] - x[i]).  F16 / BF16 store, per trace with
+// s = max|x| (1 if the trace is all zero), the pair (x[i]/s, (x[i+1]-x[i])/s)
+// rounded to nearest in the half format (low half = sample, high half =
+// difference), and scale[row][xl] = s (0 for the zero trace and row).
+// One wave per trace.
+template <int ST>
+__global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, void* __restrict__ xp,
+                                                   float* __restrict__ scale, int n_rows,
+                                                   int n_xd, int n_t) {
+  const long long wave = (long long)blockIdx.x * 4 + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const long long n_tr = (long long)(n_rows + 1) * (n_xd + 1);
+  if (wave >= n_tr) return;
+  const int row = (int)(wave / (n_xd + 1)), xl = (int)(wave % (n_xd + 1));
+  const bool real = row < n_rows && xl < n_xd;
+  const float* src = real ? x + ((size_t)row * n_xd + xl) * n_t : nullptr;
+  float s = 1.0f;
+  if constexpr (ST != PSM_STORE_F32) {
+    float m = 0.0f;
+    if (real)
+      for (int i = lane; i < n_t; i += kWave) m = fmaxf(m, fabsf(src[i]));
+    m = wave_max(m);
+    s = (m > 0.0f) ? m : 1.0f;
+    if (lane == 0) scale[wave] = real ? s : 0.0f;
+  }
+  const float inv = 1.0f / s;
+  for (int i = lane; i < n_t; i += kWave) {
+    float lo = 0.0f, d = 0.0f;
+    if (real && i < n_t - 1) {
+      lo = src[i];
+      d = src[i + 1] - lo;
     }
-    xp[idx] = pr;
+    const size_t o = (size_t)wave * n_t + i;
+    if constexpr (ST == PSM_STORE_F32) {
+      static_cast<float2*>(xp)[o] = make_float2(lo, d);
+    } else if constexpr (ST == PSM_STORE_F16) {
+      const _Float16 hl = (_Float16)(lo * inv), hd = (_Float16)(d * inv);
+      static_cast<unsigned*>(xp)[o] = (unsigned)__builtin_bit_cast(unsigned short, hl) |
+                                      ((unsigned)__builtin_bit_cast(unsigned short, hd) << 16);
+    } else {
+      // round to nearest even into bfloat16
+      auto bf = [](float v) -> unsigned {
+        const unsigned u = __float_as_uint(v);
+        return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+      };
+      static_cast<unsigned*>(xp)[o] = bf(lo * inv) | (bf(d * inv) << 16);
+    }
   }
 }
 
 }  // namespace
 
-size_t pairs_elems(int n_rows, int n_xd, int n_t) {
-  return (size_t)(n_rows + 1) * (n_xd + 1) * n_t;
+size_t pairs_bytes(int storage, int n_rows, int n_xd, int n_t) {
+  const size_t per = storage == PSM_STORE_F32 ? 8 : 4;
+  return (size_t)(n_rows + 1) * (n_xd + 1) * n_t * per;
 }
 
-hipError_t launch_pack_pairs(const float* x, float2* xp, int n_rows, int n_xd, int n_t,
-                             hipStream_t stream) {
-  const size_t n = pairs_elems(n_rows, n_xd, n_t);
-  if (n == 0) return hipSuccess;
-  const int threads = 256;
-  size_t blocks = (n + threads - 1) / threads;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(pack_pairs_kernel, dim3((unsigned)blocks), dim3(threads), 0, stream, x, xp,
-                     n_rows, n_xd, n_t);
+hipError_t launch_pack(int storage, const float* x, void* xp, float* scale, int n_rows, int n_xd,
+                       int n_t, hipStream_t stream) {
+  const long long n_tr = (long long)(n_rows + 1) * (n_xd + 1);
+  const long long blocks = (n_tr + 3) / 4;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  switch (storage) {
+    case PSM_STORE_F16:
+      hipLaunchKernelGGL(pack_kernel<PSM_STORE_F16>, dim3((unsigned)blocks), dim3(256), 0, stream,
+                         x, xp, scale, n_rows, n_xd, n_t);
+      break;
+    case PSM_STORE_BF16:
+      hipLaunchKernelGGL(pack_kernel<PSM_STORE_BF16>, dim3((unsigned)blocks), dim3(256), 0, stream,
+                         x, xp, scale, n_rows, n_xd, n_t);
+      break;
+    default:
+      hipLaunchKernelGGL(pack_kernel<PSM_STORE_F32>, dim3((unsigned)blocks), dim3(256), 0, stream,
+                         x, xp, scale, n_rows, n_xd, n_t);
+      break;
+  }
   return hipGetLastError();
 }
 
-template <int WPB>
+template <int ST, int WPB>
 hipError_t launch_wpb(const MigArgs& a, bool count_only, hipStream_t stream) {
   const int n_tiles = (a.n_t + kWave - 1) / kWave;
   const int n_jgroups = (a.n_out_xl + WPB - 1) / WPB;
@@ -406,16 +525,27 @@ hipError_t launch_wpb(const MigArgs& a, bool count_only, hipStream_t stream) {
   const int ptp = (int)per_tile_pad;
   if (count_only) {
     if (generic)
-      hipLaunchKernelGGL((migrate_kernel<WPB, true, true>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
+      hipLaunchKernelGGL((migrate_kernel<ST, WPB, true, true>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
     else
-      hipLaunchKernelGGL((migrate_kernel<WPB, false, true>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
+      hipLaunchKernelGGL((migrate_kernel<ST, WPB, false, true>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
   } else {
     if (generic)
-      hipLaunchKernelGGL((migrate_kernel<WPB, true, false>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
+      hipLaunchKernelGGL((migrate_kernel<ST, WPB, true, false>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
     else
-      hipLaunchKernelGGL((migrate_kernel<WPB, false, false>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
+      hipLaunchKernelGGL((migrate_kernel<ST, WPB, false, false>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
   }
   return hipGetLastError();
+}
+
+template <int ST>
+hipError_t launch_st(const MigArgs& a, bool count_only, hipStream_t stream, int wpb) {
+  switch (wpb) {
+    case 1: return launch_wpb<ST, 1>(a, count_only, stream);
+    case 2: return launch_wpb<ST, 2>(a, count_only, stream);
+    case 8: return launch_wpb<ST, 8>(a, count_only, stream);
+    case 16: return launch_wpb<ST, 16>(a, count_only, stream);
+    default: return launch_wpb<ST, 4>(a, count_only, stream);
+  }
 }
 
 hipError_t launch_migrate(const MigArgs& a0, bool count_only, hipStream_t stream,
@@ -430,12 +560,14 @@ hipError_t launch_migrate(const MigArgs& a0, bool count_only, hipStream_t stream
     a.d_hi = lo + R;
     a.accumulate = pass > 0 ? 1 : 0;
     hipError_t e;
-    switch (tune.waves_per_block) {
-      case 1: e = launch_wpb<1>(a, count_only, stream); break;
-      case 2: e = launch_wpb<2>(a, count_only, stream); break;
-      case 8: e = launch_wpb<8>(a, count_only, stream); break;
-      case 16: e = launch_wpb<16>(a, count_only, stream); break;
-      default: e = launch_wpb<4>(a, count_only, stream); break;
+    if (count_only) {
+      e = launch_st<PSM_STORE_F32>(a, true, stream, tune.waves_per_block);
+    } else {
+      switch (a.storage) {
+        case PSM_STORE_F16: e = launch_st<PSM_STORE_F16>(a, false, stream, tune.waves_per_block); break;
+        case PSM_STORE_BF16: e = launch_st<PSM_STORE_BF16>(a, false, stream, tune.waves_per_block); break;
+        default: e = launch_st<PSM_STORE_F32>(a, false, stream, tune.waves_per_block); break;
+      }
     }
     if (e != hipSuccess) return e;
   }
