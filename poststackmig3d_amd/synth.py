@@ -91,7 +91,7 @@ def production(n_il=401, n_xl=401, n_t=1001, dt=0.002, dx=25.0, dy=25.0, f=30.0,
     for _ in range(n_diffractors):
         ia = rng.integers(0, n_il)
         xa = rng.integers(0, n_xl)
-        ta = rng.uniform(0.2, (n_t - 1) * dt * 0.9)
+        ta = rng.uniform(min(0.2, 0.3 * (n_t - 1) * dt), (n_t - 1) * dt * 0.9)
         va = float(vel[ia, xa, min(int(ta / dt), n_t - 1)])
         r2 = (X - ia * dx) ** 2 + (Y - xa * dy) ** 2
         tau = np.sqrt(ta * ta + 4.0 * r2 / (va * va))
@@ -138,7 +138,7 @@ def production_torch(device, n_il=401, n_xl=401, n_t=1001, dt=0.002, dx=25.0, dy
     for _ in range(n_diffractors):
         ia = rng.integers(0, n_il)
         xa = rng.integers(0, n_xl)
-        ta = rng.uniform(0.2, (n_t - 1) * dt * 0.9)
+        ta = rng.uniform(min(0.2, 0.3 * (n_t - 1) * dt), (n_t - 1) * dt * 0.9)
         va = float(velc[ia, xa, min(int(ta / dt), n_t - 1)])
         r2 = (X - ia * dx) ** 2 + (Y - xa * dy) ** 2
         add(torch.sqrt(ta * ta + 4.0 * r2 / (va * va)), rng.uniform(0.5, 1.0))
