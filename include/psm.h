@@ -141,6 +141,37 @@ int psm_migrate_volume(psm_ctx* ctx, const psm_survey* survey,
                        const float* data, const float* vel, float* out,
                        int where);
 
+/*
+ * Adjoint of psm_migrate_volume (Kirchhoff demigration / modelling; no
+ * reference counterpart): image [n_out_il][n_out_xl][n_t] -> data
+ * [n_il][n_xl][n_t].  Walks the same pairs with the same sample index,
+ * fraction, weight and mask as the forward operator and scatters
+ * d[i] += w (1 - f) m, d[i+1] += w f m, so <M x, y> = <x, M^T y> up to float
+ * rounding.  Crossline padding traces are zero inputs of the forward
+ * operator and receive nothing.  Memory per `where`, as psm_migrate_volume.
+ */
+int psm_demigrate_volume(psm_ctx* ctx, const psm_survey* survey,
+                         const float* image, const float* vel, float* data,
+                         int where);
+
+/*
+ * Velocity scale (no reference counterpart): later volume calls on this
+ * context use alpha * Vrms for travel times and weights and
+ * alpha_mask * Vrms for the aperture mask, its crossline window and the
+ * inline window (1, 1 = the reference operator).  psm_migrate_volume_dalpha
+ * returns d(image)/d(alpha) at the current scales with the mask held at
+ * alpha_mask, the tangent used for velocity-scale gradients.
+ */
+int psm_set_velocity_scale(psm_ctx* ctx, float alpha, float alpha_mask);
+int psm_migrate_volume_dalpha(psm_ctx* ctx, const psm_survey* survey,
+                              const float* data, const float* vel, float* out,
+                              int where);
+
+/* sum x[i] y[i] accumulated in float64 on the GPU (deterministic order);
+ * x, y host or device per `where`. */
+int psm_dot_f64(psm_ctx* ctx, const float* x, const float* y, size_t n, int where,
+                double* result);
+
 /* Pair counts of the work unit for a survey (no reference counterpart):
  * p_ap = (output sample, real aperture trace) pairs inside the aperture
  * mask, p_g = those that also gather (i < n_t - 1).  ap_image / g_image

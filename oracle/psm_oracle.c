@@ -133,6 +133,24 @@ PSMO_API int psmo_kernel_inline_f32(
     return 0;
 }
 
+/* The float32 decision of migration_kernel.cpp:298 (i < n_t - 1) for one
+ * pair, so that the double-precision oracles sum the same pair set; returns
+ * the double-precision (i, f), clamped to the last interval when the two
+ * precisions straddle the bound. */
+static int bound_f32(float vf, float t0f, float r2f, float inv_dtf, int n_t,
+                     double q, int* i_out, double* f_out)
+{
+    const float af = 4.0f * (1.0f / (vf * vf + 1e-10f));
+    const float qf = sqrtf(fmaf(af, r2f, t0f * t0f)) * inv_dtf;
+    if ((int)qf >= n_t - 1) return 0;
+    int i = (int)q;
+    double f = q - (double)i;
+    if (i > n_t - 2) { i = n_t - 2; f = 1.0; }
+    *i_out = i;
+    *f_out = f;
+    return 1;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Per-call kernel, double precision (accuracy yardstick, not a reference   */
 /* function).  The discrete decisions -- sample validity, the aperture mask */
@@ -153,6 +171,7 @@ PSMO_API int psmo_kernel_inline_f64(
     if (n_xl == 0 || n_t == 0) return 0;
     const size_t stride_il = n_buf_xl * n_t;
     const float dxf = (float)dx, dyf = (float)dy, inv_dy = 1.0f / dyf;
+    const float inv_dtf = 1.0f / (float)dt;
     const int32_t cpos = (cur_il_pos >= 0 && (size_t)cur_il_pos < n_il_pos)
                              ? il_pos[cur_il_pos] : 0;
     float* dx2f = (float*)malloc(sizeof(float) * (n_buf_il ? n_buf_il : 1));
@@ -198,14 +217,14 @@ PSMO_API int psmo_kernel_inline_f64(
                 const float* row = in + b * stride_il;
                 for (int k = c_lo[s]; k < c_hi[s]; ++k) {
                     const float dyv = (float)(k - kj) * dyf;
-                    if (fmaf(dyv, dyv, dx2f[b]) > R2) continue;
+                    const float r2f = fmaf(dyv, dyv, dx2f[b]);
+                    if (r2f > R2) continue;
                     const double dyd = (double)(k - kj) * dy;
                     const double r2 = dyd * dyd + dx2[b];
                     const double tau = sqrt(t0 * t0 + a * r2);
-                    const double q = tau / dt;
-                    const int i = (int)q;
-                    if (i < (int)n_t - 1) {
-                        const double f = q - (double)i;
+                    int i;
+                    double f;
+                    if (bound_f32((float)v, t0_times[s], r2f, inv_dtf, (int)n_t, tau / dt, &i, &f)) {
                         const float* tr = row + (size_t)k * n_t;
                         double val = (double)tr[i] + f * ((double)tr[i + 1] - (double)tr[i]);
                         if (amp) val /= tau * v * v;
@@ -446,6 +465,7 @@ PSMO_API int psmo_adjoint_volume_f64(const double* image, const float* vel,
     psmo_aperture(g->angle_deg, vmax, n_t, g->dt, g->il_step, &T2, &ap);
     const double dt = g->dt, dx = g->il_step, dy = g->xl_step;
     const float dxf = (float)dx, dyf = (float)dy, inv_dy = 1.0f / dyf;
+    const float inv_dtf = 1.0f / (float)dt;
     memset(data_out, 0, sizeof(double) * n_il * n_xl * n_t);
     for (size_t io = 0; io < n_out_il; ++io) {
         const size_t c = psmo_map_output_inline(ill, n_il, oll[io]);
@@ -487,10 +507,11 @@ PSMO_API int psmo_adjoint_volume_f64(const double* image, const float* vel,
                         const double dyd = (double)(k - kj) * dy;
                         const double r2 = dyd * dyd + dxd * dxd;
                         const double tau = sqrt(t0 * t0 + a * r2);
-                        const double q = tau / dt;
-                        const int i = (int)q;
-                        if (i >= (int)n_t - 1) continue;
-                        const double f = q - (double)i;
+                        int i;
+                        double f;
+                        if (!bound_f32(vf, t0f, fmaf(dyv, dyv, d2f), inv_dtf, (int)n_t, tau / dt,
+                                       &i, &f))
+                            continue;
                         double wgt = m;
                         if (g->amp) wgt /= tau * v * v;
                         double* tr = data_out + ((size_t)b * n_xl + x) * n_t;

@@ -8,12 +8,16 @@ from ._lib import PsmError, context, load  # noqa: F401
 from .kernel import (  # noqa: F401
     SurveyParams,
     count_pairs,
+    demigrate_volume,
+    dot_f64,
     kirchhoff_kernel_3d_with_padding,
     migrate_volume,
+    migrate_volume_dalpha,
     survey_plan,
 )
 
 __all__ = [
-    "PsmError", "context", "load", "SurveyParams", "count_pairs",
-    "kirchhoff_kernel_3d_with_padding", "migrate_volume", "survey_plan",
+    "PsmError", "context", "load", "SurveyParams", "count_pairs", "demigrate_volume",
+    "dot_f64", "kirchhoff_kernel_3d_with_padding", "migrate_volume", "migrate_volume_dalpha",
+    "survey_plan",
 ]

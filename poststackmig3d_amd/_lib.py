@@ -75,6 +75,13 @@ SIGNATURES = [
     ("psm_count_pairs", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p, C.c_int,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p,
                                   C.c_void_p]),
+    ("psm_demigrate_volume", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_int]),
+    ("psm_set_velocity_scale", C.c_int, [C.c_void_p, C.c_float, C.c_float]),
+    ("psm_migrate_volume_dalpha", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_int]),
+    ("psm_dot_f64", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                              C.POINTER(C.c_double)]),
     ("psm_set_storage", C.c_int, [C.c_void_p, C.c_int]),
     ("psm_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("psm_get_tuning", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -144,6 +151,13 @@ class Context:
         if storage not in STORAGE:
             raise PsmError(f"unknown storage {storage!r}")
         self.check(self.lib.psm_set_storage(self.h, STORAGE[storage]), "psm_set_storage")
+
+    def set_velocity_scale(self, alpha: float = 1.0, alpha_mask: float | None = None):
+        """Travel times and weights use alpha * Vrms, the aperture mask
+        alpha_mask * Vrms (default: alpha)."""
+        am = alpha if alpha_mask is None else alpha_mask
+        self.check(self.lib.psm_set_velocity_scale(self.h, float(alpha), float(am)),
+                   "psm_set_velocity_scale")
 
     def set_tuning(self, waves_per_block: int = 4, rows_per_pass: int = 0):
         self.check(self.lib.psm_set_tuning(self.h, int(waves_per_block), int(rows_per_pass)),

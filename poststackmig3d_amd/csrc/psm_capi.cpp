@@ -51,13 +51,14 @@ struct psm_ctx {
   hipStream_t stream = nullptr;
   std::mutex mu;
   std::string err;
-  DevBuf raw, pairs, scale, vel, out, t0, tasks, rowpos, counters;
+  DevBuf raw, pairs, scale, vel, out, t0, tasks, rowpos, counters, dot, dotx, doty;
   int storage = PSM_STORE_F32;
   // kernel-level timing of the last volume call (psm_enable_timing)
   bool timing = false;
   bool timed = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   psm::Tuning tune;
+  float alpha = 1.0f, alpha_mask = 1.0f;
 };
 
 namespace {
@@ -150,7 +151,7 @@ int check_survey(psm_ctx* c, const psm_survey* s) {
 }
 
 int plan_impl(psm_ctx* c, const psm_survey* s, const float* vel_probe, int where,
-              psm_plan* plan) {
+              psm_plan* plan, float alpha_mask = 1.0f) {
   int rc = check_survey(c, s);
   if (rc) return rc;
   if (!plan) return fail(c, PSM_ERR_ARG, "plan is NULL");
@@ -184,6 +185,9 @@ int plan_impl(psm_ctx* c, const psm_survey* s, const float* vel_probe, int where
       }
     if (vmax < 100.0f) vmax = 3000.0f;
   }
+  // a scaled velocity model scales the probed maximum (exact for the
+  // reference operator, alpha_mask = 1)
+  vmax *= alpha_mask;
   // main.cpp:188-202
   const float tan_theta = (float)std::tan(s->angle_aperture * 3.14159265358979323846 / 180.0);
   const float tan2 = tan_theta * tan_theta;
@@ -257,7 +261,8 @@ void psm_destroy(psm_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (DevBuf* b : {&c->raw, &c->pairs, &c->scale, &c->vel, &c->out, &c->t0, &c->tasks, &c->rowpos, &c->counters})
+  for (DevBuf* b : {&c->raw, &c->pairs, &c->scale, &c->vel, &c->out, &c->t0, &c->tasks,
+                    &c->rowpos, &c->counters, &c->dot, &c->dotx, &c->doty})
     b->release();
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -358,6 +363,8 @@ int psm_kirchhoff_inline(psm_ctx* c, const float* input_buffer, size_t n_buf_il,
   a.inv_dy = 1.0f / crossline_bin_size;      // :45
   a.tan2 = tan_theta_sq;
   a.amp = amp_correction ? 1 : 0;
+  a.alpha = c->alpha;
+  a.alpha_mask = c->alpha_mask;
   if (n_buf_il == 0) {
     // no aperture traces: every valid sample sums to zero
     a.tasks = c->tasks.as<psm::Task>();
@@ -367,7 +374,7 @@ int psm_kirchhoff_inline(psm_ctx* c, const float* input_buffer, size_t n_buf_il,
     extent = (int)n_buf_il - 1;
   else if (n_buf_il)
     extent = (int)std::max<long long>(0, std::max<long long>((long long)n_buf_il - 1 - crow, crow));
-  PSM_HIP(c, psm::launch_migrate(a, false, st, c->tune, extent));
+  PSM_HIP(c, psm::launch_migrate(a, psm::kForward, st, c->tune, extent));
   PSM_HIP(c, hipMemcpyAsync(output_slice, c->out.p, n_xl * n_t * sizeof(float), hipMemcpyDeviceToHost, st));
   PSM_HIP(c, hipStreamSynchronize(st));
   return PSM_OK;
@@ -391,15 +398,22 @@ void psm_plan_free(psm_plan* p) {
 
 namespace {
 
-// Shared body of psm_migrate_volume / psm_count_pairs (lock held).
-int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float* vel,
-                float* out, int where, bool count, double* p_ap, double* p_g,
+// Shared body of the volume entry points (lock held).  mode:
+//   kForward  in = data cube,  out = image
+//   kTangent  in = data cube,  out = d image / d alpha
+//   kAdjoint  in = image,      out = data cube
+//   kCount    in unused,       out unused; counts (and count images)
+int volume_impl(psm_ctx* c, const psm_survey* s, const float* in, const float* vel, float* out,
+                int where, int mode, double* p_ap = nullptr, double* p_g = nullptr,
                 float* ap_image = nullptr, float* g_image = nullptr) {
+  const bool count = mode == psm::kCount;
+  const bool adjoint = mode == psm::kAdjoint;
+  const bool gathers = mode == psm::kForward || mode == psm::kTangent;
   if (where != PSM_HOST && where != PSM_DEVICE) return fail(c, PSM_ERR_ARG, "bad `where`");
-  if (!vel || (!count && (!data || !out))) return fail(c, PSM_ERR_ARG, "null data/vel/out pointer");
+  if (!vel || (!count && (!in || !out))) return fail(c, PSM_ERR_ARG, "null input/vel/out pointer");
   PSM_HIP(c, hipSetDevice(c->device));
   psm_plan plan;
-  int rc = plan_impl(c, s, vel, where, &plan);
+  int rc = plan_impl(c, s, vel, where, &plan, c->alpha_mask);
   if (rc) return rc;
   struct PlanGuard {
     psm_plan* p;
@@ -408,8 +422,10 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
   hipStream_t st = c->stream;
   const size_t n_tr = s->n_il * s->n_xl, nt = s->n_t;
   const size_t cube = n_tr * nt;
+  const size_t out_elems = plan.n_out_il * plan.n_out_xl * nt;
   const int padx = s->crossline_padding;
   const size_t K = s->n_xl + 2 * (size_t)padx;
+  const int storage = mode == psm::kForward ? c->storage : PSM_STORE_F32;
 
   std::vector<psm::Task> tasks(plan.n_out_il);
   int extent = 0;
@@ -421,46 +437,47 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
   PSM_HIP(c, upload(c->tasks, tasks.data(), tasks.size() * sizeof(psm::Task), st));
   PSM_HIP(c, upload(c->t0, plan.t0_times, nt * sizeof(float), st));
 
+  // device views of the inputs and outputs
   const float* d_vel = vel;
-  const float* d_data = data;
+  const float* d_in = in;
+  float* d_out = out;
+  const size_t in_elems = adjoint ? out_elems : cube;
+  const size_t res_elems = adjoint ? cube : out_elems;
   if (where == PSM_HOST) {
     PSM_HIP(c, upload(c->vel, vel, cube * sizeof(float), st));
     d_vel = c->vel.as<float>();
     if (!count) {
-      PSM_HIP(c, upload(c->raw, data, cube * sizeof(float), st));
-      d_data = c->raw.as<float>();
+      PSM_HIP(c, upload(c->raw, in, in_elems * sizeof(float), st));
+      d_in = c->raw.as<float>();
+      PSM_ALLOC(c, c->out, res_elems * sizeof(float));
+      d_out = c->out.as<float>();
     }
   }
   if (((size_t)s->n_xl + 1) * nt * sizeof(float2) >= (size_t)INT32_MAX)
     return fail(c, PSM_ERR_ARG, "one inline of the survey exceeds 2 GiB");
-  if (!count) {
-    PSM_ALLOC(c, c->pairs, psm::pairs_bytes(c->storage, (int)s->n_il, (int)s->n_xl, (int)nt));
+  if (gathers) {
+    PSM_ALLOC(c, c->pairs, psm::pairs_bytes(storage, (int)s->n_il, (int)s->n_xl, (int)nt));
     PSM_ALLOC(c, c->scale, (s->n_il + 1) * (s->n_xl + 1) * sizeof(float));
   }
+  const bool timed = c->timing && mode == psm::kForward;
   c->timed = false;
-  if (c->timing && !count) {
+  if (timed) {
     for (hipEvent_t& e : c->ev)
       if (!e) PSM_HIP(c, hipEventCreate(&e));
     PSM_HIP(c, hipEventRecord(c->ev[0], st));
   }
-  if (!count)
-    PSM_HIP(c, psm::launch_pack(c->storage, d_data, c->pairs.p, c->scale.as<float>(), (int)s->n_il,
+  if (gathers)
+    PSM_HIP(c, psm::launch_pack(storage, d_in, c->pairs.p, c->scale.as<float>(), (int)s->n_il,
                                 (int)s->n_xl, (int)nt, st));
-  if (c->timing && !count) PSM_HIP(c, hipEventRecord(c->ev[1], st));
+  if (timed) PSM_HIP(c, hipEventRecord(c->ev[1], st));
 
-  float* d_out = out;
-  const size_t out_elems = plan.n_out_il * plan.n_out_xl * nt;
-  if (!count && where == PSM_HOST) {
-    PSM_ALLOC(c, c->out, out_elems * sizeof(float));
-    d_out = c->out.as<float>();
-  }
   psm::MigArgs a{};
   a.xp = c->pairs.p;
   a.scale = c->scale.as<float>();
   a.n_rows = (int)s->n_il;
-  a.storage = c->storage;
+  a.storage = storage;
   a.vel = d_vel;
-  a.out = d_out;
+  a.out = gathers ? d_out : nullptr;
   a.t0 = c->t0.as<float>();
   a.tasks = c->tasks.as<psm::Task>();
   a.row_pos = nullptr;
@@ -481,6 +498,13 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
   a.inv_dy = 1.0f / a.dy;
   a.tan2 = plan.tan_theta_sq;
   a.amp = s->amp_correction ? 1 : 0;
+  a.alpha = c->alpha;
+  a.alpha_mask = c->alpha_mask;
+  if (adjoint) {
+    a.image = d_in;
+    a.adj_out = d_out;
+    PSM_HIP(c, psm::launch_fill_zero(d_out, cube, st));
+  }
   if (count) {
     PSM_ALLOC(c, c->counters, 2 * sizeof(unsigned long long));
     PSM_HIP(c, hipMemsetAsync(c->counters.p, 0, 2 * sizeof(unsigned long long), st));
@@ -497,7 +521,7 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
         a.out_g = g_image ? c->out.as<float>() + out_elems : nullptr;
       }
     }
-    PSM_HIP(c, psm::launch_migrate(a, true, st, c->tune, extent));
+    PSM_HIP(c, psm::launch_migrate(a, psm::kCount, st, c->tune, extent));
     unsigned long long h[2] = {0, 0};
     PSM_HIP(c, hipMemcpyAsync(h, c->counters.p, sizeof h, hipMemcpyDeviceToHost, st));
     if (where == PSM_HOST) {
@@ -511,13 +535,13 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* data, const float*
     if (p_g) *p_g = (double)h[1];
     return PSM_OK;
   }
-  PSM_HIP(c, psm::launch_migrate(a, false, st, c->tune, extent));
-  if (c->timing) {
+  PSM_HIP(c, psm::launch_migrate(a, mode, st, c->tune, extent));
+  if (timed) {
     PSM_HIP(c, hipEventRecord(c->ev[2], st));
     c->timed = true;
   }
   if (where == PSM_HOST) {
-    PSM_HIP(c, hipMemcpyAsync(out, d_out, out_elems * sizeof(float), hipMemcpyDeviceToHost, st));
+    PSM_HIP(c, hipMemcpyAsync(out, d_out, res_elems * sizeof(float), hipMemcpyDeviceToHost, st));
     PSM_HIP(c, hipStreamSynchronize(st));
   }
   return PSM_OK;
@@ -531,14 +555,61 @@ int psm_migrate_volume(psm_ctx* c, const psm_survey* s, const float* data, const
                        float* out, int where) {
   if (!c) return PSM_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
-  return volume_impl(c, s, data, vel, out, where, false, nullptr, nullptr);
+  return volume_impl(c, s, data, vel, out, where, psm::kForward);
+}
+
+int psm_demigrate_volume(psm_ctx* c, const psm_survey* s, const float* image, const float* vel,
+                         float* data, int where) {
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return volume_impl(c, s, image, vel, data, where, psm::kAdjoint);
+}
+
+int psm_migrate_volume_dalpha(psm_ctx* c, const psm_survey* s, const float* data,
+                              const float* vel, float* out, int where) {
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return volume_impl(c, s, data, vel, out, where, psm::kTangent);
+}
+
+int psm_dot_f64(psm_ctx* c, const float* x, const float* y, size_t n, int where, double* result) {
+  if (!c || !result) return PSM_ERR_ARG;
+  if (n && (!x || !y)) return fail(c, PSM_ERR_ARG, "null vector");
+  if (where != PSM_HOST && where != PSM_DEVICE) return fail(c, PSM_ERR_ARG, "bad `where`");
+  std::lock_guard<std::mutex> lk(c->mu);
+  PSM_HIP(c, hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const float* dx = x;
+  const float* dy = y;
+  if (where == PSM_HOST && n) {
+    PSM_HIP(c, upload(c->dotx, x, n * sizeof(float), st));
+    PSM_HIP(c, upload(c->doty, y, n * sizeof(float), st));
+    dx = c->dotx.as<float>();
+    dy = c->doty.as<float>();
+  }
+  PSM_ALLOC(c, c->dot, psm::dot_scratch_doubles() * sizeof(double));
+  double* part = c->dot.as<double>();
+  PSM_HIP(c, psm::launch_dot_f64(dx, dy, n, part, part + psm::dot_scratch_doubles() - 1, st));
+  PSM_HIP(c, hipMemcpyAsync(result, part + psm::dot_scratch_doubles() - 1, sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+  PSM_HIP(c, hipStreamSynchronize(st));
+  return PSM_OK;
+}
+
+int psm_set_velocity_scale(psm_ctx* c, float alpha, float alpha_mask) {
+  if (!c) return PSM_ERR_ARG;
+  if (!(alpha > 0) || !(alpha_mask > 0)) return fail(c, PSM_ERR_ARG, "velocity scales must be > 0");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->alpha = alpha;
+  c->alpha_mask = alpha_mask;
+  return PSM_OK;
 }
 
 int psm_count_pairs(psm_ctx* c, const psm_survey* s, const float* vel, int where, double* p_ap,
                     double* p_g, float* ap_image, float* g_image) {
   if (!c) return PSM_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
-  return volume_impl(c, s, nullptr, vel, nullptr, where, true, p_ap, p_g, ap_image, g_image);
+  return volume_impl(c, s, nullptr, vel, nullptr, where, psm::kCount, p_ap, p_g, ap_image, g_image);
 }
 
 int psm_set_storage(psm_ctx* c, int storage) {

@@ -11,6 +11,14 @@
 
 namespace psm {
 
+// Kernel modes.
+enum Mode : int {
+  kForward = 0,  // migration: out = sum w lerp(x, q)
+  kCount = 1,    // pair counts of the forward sum
+  kAdjoint = 2,  // demigration: scatter the image into the data cube
+  kTangent = 3,  // d(out)/d(alpha) for a velocity scale alpha, mask fixed
+};
+
 // One output inline of work: the centre row (in data-row index space), the
 // data-row window [b0, b1) it sums over, and the velocity row it reads.
 struct Task {
@@ -42,6 +50,10 @@ struct MigArgs {
   int amp;
   int d_lo, d_hi;          // this pass: row offsets |db| in [d_lo, d_hi) (generic rows: row indices)
   int accumulate;          // 0: out = sum; 1: out += sum (later passes)
+  float alpha;             // velocity scale of travel time and weight (1 = reference)
+  float alpha_mask;        // velocity scale of the aperture mask and window (1 = reference)
+  const float* image;      // adjoint: image [n_out_il][n_out_xl][n_t] to scatter
+  float* adj_out;          // adjoint: data [n_rows][n_xd][n_t], accumulated (zeroed by the caller)
 };
 
 // Launch-shape parameters (chosen by the autotuner or the caller).
@@ -61,7 +73,12 @@ hipError_t launch_pack(int storage, const float* x, void* xp, float* scale, int 
 // row_extent: largest row offset (generic rows: largest row index) any task
 // can reach; the launcher splits [0, row_extent] into passes of
 // tune.rows_per_pass row offsets.
-hipError_t launch_migrate(const MigArgs& a, bool count_only, hipStream_t stream,
-                          const Tuning& tune, int row_extent);
+hipError_t launch_migrate(const MigArgs& a, int mode, hipStream_t stream, const Tuning& tune,
+                          int row_extent);
+hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream);
+// out[0] = sum x[i] y[i] in float64; partial holds dot_scratch_doubles() - 1 doubles.
+size_t dot_scratch_doubles();
+hipError_t launch_dot_f64(const float* x, const float* y, size_t n, double* partial, double* out,
+                          hipStream_t stream);
 
 }  // namespace psm

@@ -142,6 +142,7 @@ struct Store<PSM_STORE_BF16> {
 struct LaneParams {
   float t0sq, a, R2, v2;
   int wlo, whi;
+  float mval;  // adjoint: the image sample this lane scatters
 };
 
 // One (|db|, |dk|) step of the aperture sum: the fraction, weight and mask
@@ -159,16 +160,20 @@ struct Step {
 // |db| in [d_lo, d_hi) of this pass.  WINDOW = some lane uses parameters of
 // an earlier crossline whose crossline window can cut the aperture (sticky
 // samples of migration_kernel.cpp:64-152), so each gather is also checked
-// against that window.  COUNT = count pairs instead of summing.  All loop
+// against that window.  MODE: kForward sums, kCount counts pairs, kAdjoint
+// scatters the lane's image sample into the data cube with the same
+// (i, f, w, mask) as the forward sum, kTangent sums the derivative of each
+// term with respect to the velocity scale.  All loop
 // bounds and trace offsets are wave-uniform (scalar registers); absent
 // crosslines read the row's zero trace and absent rows the zero row, so the
 // four gathers of a step are issued back to back, and the fast path loads
 // step dk+1 before it consumes step dk (two steps of gathers in flight).
-template <int ST, bool GENERIC_ROWS, bool WINDOW, bool COUNT>
+template <int ST, bool GENERIC_ROWS, bool WINDOW, int MODE>
 __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, int kj,
                                              const LaneParams& p, float R2max,
                                              float& acc, unsigned& cap, unsigned& cg) {
   using S = Store<ST>;
+  constexpr bool COUNT = MODE == kCount;
   const int n_t = a.n_t;
   const int nt1 = n_t - 1;
   const int xd_lo = a.xd_off, xd_hi = a.xd_off + a.n_xd;
@@ -200,7 +205,7 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
     const int zero_off = a.n_xd * tb;  // the row's zero trace
     const unsigned nxd = (unsigned)a.n_xd;
 
-    if constexpr (!WINDOW && !COUNT) {
+    if constexpr (!WINDOW && MODE == kForward) {
       // Fast path.  Each row is read through a buffer resource whose base is
       // the row; a gather's address is (scalar trace offset) + (lane sample
       // offset).
@@ -288,15 +293,39 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
           const unsigned n = nrow * ((has_kp && okp ? 1u : 0u) + (has_km && okm ? 1u : 0u));
           cap += in_mask ? n : 0u;
           cg += (in_mask && i < nt1) ? n : 0u;
+        } else if constexpr (MODE == kAdjoint) {
+          // d[b][k][i] += w (1 - f) m,  d[b][k][i+1] += w f m  (transpose of
+          // the interpolation fma(f, x[i+1] - x[i], x[i]))
+          if (in_mask && i < nt1) {
+            const float f = __builtin_amdgcn_fractf(q);
+            const float w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+            const float wm = w * p.mval;
+            const float c0 = wm - wm * f, c1 = wm * f;
+            auto scatter = [&](bool row_ok, int row, bool k_ok, int k) {
+              if (row_ok && k_ok) {
+                float* d = a.adj_out + ((size_t)row * a.n_xd + (k - xd_lo)) * n_t + i;
+                unsafeAtomicAdd(d, c0);
+                unsafeAtomicAdd(d + 1, c1);
+              }
+            };
+            scatter(hp, ip, has_kp && okp, kp);
+            scatter(hp, ip, has_km && okm, km);
+            scatter(hm, im, has_kp && okp, kp);
+            scatter(hm, im, has_km && okm, km);
+          }
         } else {
           const int xp_ = has_kp ? kp - xd_lo : a.n_xd;
           const int xm_ = has_km ? km - xd_lo : a.n_xd;
           const size_t off = (size_t)min(i, nt1) * S::kBytes;
           const float f = __builtin_amdgcn_fractf(q);
-          float l0 = S::lerp(S::load_ptr(rowp + (size_t)xp_ * tb + off), f);
-          float l1 = S::lerp(S::load_ptr(rowp + (size_t)xm_ * tb + off), f);
-          float l2 = S::lerp(S::load_ptr(rowm + (size_t)xp_ * tb + off), f);
-          float l3 = S::lerp(S::load_ptr(rowm + (size_t)xm_ * tb + off), f);
+          const typename S::Raw x0 = S::load_ptr(rowp + (size_t)xp_ * tb + off);
+          const typename S::Raw x1 = S::load_ptr(rowp + (size_t)xm_ * tb + off);
+          const typename S::Raw x2 = S::load_ptr(rowm + (size_t)xp_ * tb + off);
+          const typename S::Raw x3 = S::load_ptr(rowm + (size_t)xm_ * tb + off);
+          float l0 = S::lerp(x0, f);
+          float l1 = S::lerp(x1, f);
+          float l2 = S::lerp(x2, f);
+          float l3 = S::lerp(x3, f);
           if constexpr (S::kScaled) {
             l0 *= scp[xp_];
             l1 *= scp[xm_];
@@ -310,7 +339,22 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
           float sum = (l0 + l1) + (l2 + l3);
           sum = in_mask ? sum : 0.0f;
           const float w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
-          acc = __builtin_fmaf(sum, w, acc);
+          if constexpr (MODE == kTangent) {
+            // d/d(alpha) of w * lerp(x, tau/dt) with v -> alpha v and the mask
+            // held fixed: dtau = -(a r^2) / (alpha tau), dq = dtau / dt,
+            // dw = -w (dtau / tau + 2 / alpha); d lerp / dq = x[i+1] - x[i]
+            float dsum = 0.0f;
+            if constexpr (!S::kScaled) {
+              dsum = (okp ? x0.y + x2.y : 0.0f) + (okm ? x1.y + x3.y : 0.0f);
+            }
+            dsum = in_mask ? dsum : 0.0f;
+            const float dtau = -(p.a * r2) / (a.alpha * tau);
+            const float dq = dtau * a.inv_dt;
+            const float dw = a.amp ? -w * (dtau / tau + 2.0f / a.alpha) : 0.0f;
+            acc += dw * sum + w * dq * dsum;
+          } else {
+            acc = __builtin_fmaf(sum, w, acc);
+          }
         }
       }
     }
@@ -344,9 +388,11 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
   }
 }
 
-template <int ST, int WPB, bool GENERIC_ROWS, bool COUNT>
+template <int ST, int WPB, bool GENERIC_ROWS, int MODE>
 __global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_tiles,
                                                               int n_jgroups, int per_tile_pad) {
+  constexpr bool COUNT = MODE == kCount;
+  constexpr bool WRITES_OUT = MODE == kForward || MODE == kTangent;
   // ---- block -> (t0 tile, output inline, crossline group), heavy tiles first
   const int g = blockIdx.x;
   const int tile = n_tiles - 1 - g / per_tile_pad;
@@ -363,7 +409,7 @@ __global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_t
   float* orow = a.out + ((size_t)io * a.n_out_xl + j) * a.n_t;
 
   if (kj < 0 || kj >= a.K) {  // migration_kernel.cpp:79-85
-    if (!COUNT && !a.accumulate && active) orow[s] = 0.0f;
+    if (WRITES_OUT && !a.accumulate && active) orow[s] = 0.0f;
     return;
   }
   const Task tk = a.tasks[io];
@@ -385,18 +431,23 @@ __global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_t
       }
     }
   }
-  LaneParams p{0.0f, 0.0f, -1.0f, 1.0f, 0, 0};
+  LaneParams p{0.0f, 0.0f, -1.0f, 1.0f, 0, 0, 0.0f};
   bool needs_window = false;
   if (src >= 0) {
+    // velocity scale: travel time and weight use alpha v, the aperture mask
+    // and window alpha_mask v (both 1 for the reference operator, where the
+    // products are exact)
+    const float vt = a.alpha * v, vm = a.alpha_mask * v;
     p.t0sq = t0 * t0;
-    const float inv_v2 = 1.0f / (v * v + 1e-10f);
+    const float inv_v2 = 1.0f / (vt * vt + 1e-10f);
     p.a = 4.0f * inv_v2;
-    const float vth = v * t0 * 0.5f;
+    const float vth = vm * t0 * 0.5f;
     p.R2 = vth * vth * a.tan2;
     const int w = (int)(sqrt_rn(p.R2) * a.inv_dy) + 1;
     p.wlo = max(0, src - w);
     p.whi = min(a.K, src + w + 1);
-    p.v2 = v * v;
+    p.v2 = vt * vt;
+    if constexpr (MODE == kAdjoint) p.mval = a.image[((size_t)io * a.n_out_xl + j) * a.n_t + s];
     if (src != kj) {
       // crosslines this lane could reach around kj, clipped to the data
       const int lo_need = max(a.xd_off, kj - w);
@@ -409,9 +460,9 @@ __global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_t
   unsigned cap = 0, cg = 0;
   if (R2max >= 0.0f) {
     if (__any(needs_window))
-      sum_aperture<ST, GENERIC_ROWS, true, COUNT>(a, tk, kj, p, R2max, acc, cap, cg);
+      sum_aperture<ST, GENERIC_ROWS, true, MODE>(a, tk, kj, p, R2max, acc, cap, cg);
     else
-      sum_aperture<ST, GENERIC_ROWS, false, COUNT>(a, tk, kj, p, R2max, acc, cap, cg);
+      sum_aperture<ST, GENERIC_ROWS, false, MODE>(a, tk, kj, p, R2max, acc, cap, cg);
   }
   if constexpr (COUNT) {
     if (active && a.out) orow[s] = (float)cap;
@@ -421,13 +472,54 @@ __global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_t
       atomicAdd(&a.counters[0], (unsigned long long)tap);
       atomicAdd(&a.counters[1], (unsigned long long)tg);
     }
-  } else if (active) {
+  } else if (WRITES_OUT && active) {
     if (a.accumulate) {
       if (src >= 0) orow[s] += acc;
     } else {
       orow[s] = (src >= 0) ? acc : 0.0f;  // :148-151, :375
     }
   }
+}
+
+// FP64 dot product of two float32 vectors in two deterministic passes: block
+// partial sums (fixed grid, fixed tree order), then one block over the
+// partials.
+constexpr int kDotBlocks = 1024;
+
+__global__ __launch_bounds__(256) void dot_partial_kernel(const float* __restrict__ x,
+                                                          const float* __restrict__ y, size_t n,
+                                                          double* __restrict__ partial) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    s = __builtin_fma((double)x[i], (double)y[i], s);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void dot_final_kernel(const double* __restrict__ partial, int n,
+                                                        double* __restrict__ out) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+__global__ void fill_zero_kernel(float* __restrict__ p, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
 }
 
 // ---------------------------------------------------------------- packing
@@ -512,8 +604,8 @@ hipError_t launch_pack(int storage, const float* x, void* xp, float* scale, int 
   return hipGetLastError();
 }
 
-template <int ST, int WPB>
-hipError_t launch_wpb(const MigArgs& a, bool count_only, hipStream_t stream) {
+template <int ST, int WPB, int MODE>
+hipError_t launch_wpb(const MigArgs& a, hipStream_t stream) {
   const int n_tiles = (a.n_t + kWave - 1) / kWave;
   const int n_jgroups = (a.n_out_xl + WPB - 1) / WPB;
   const long long per_tile = (long long)a.n_out_il * n_jgroups;
@@ -523,55 +615,68 @@ hipError_t launch_wpb(const MigArgs& a, bool count_only, hipStream_t stream) {
   const dim3 grid((unsigned)blocks), block(kWave * WPB);
   const bool generic = a.row_pos != nullptr;
   const int ptp = (int)per_tile_pad;
-  if (count_only) {
-    if (generic)
-      hipLaunchKernelGGL((migrate_kernel<ST, WPB, true, true>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
-    else
-      hipLaunchKernelGGL((migrate_kernel<ST, WPB, false, true>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
-  } else {
-    if (generic)
-      hipLaunchKernelGGL((migrate_kernel<ST, WPB, true, false>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
-    else
-      hipLaunchKernelGGL((migrate_kernel<ST, WPB, false, false>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
-  }
+  if (generic)
+    hipLaunchKernelGGL((migrate_kernel<ST, WPB, true, MODE>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
+  else
+    hipLaunchKernelGGL((migrate_kernel<ST, WPB, false, MODE>), grid, block, 0, stream, a, n_tiles, n_jgroups, ptp);
   return hipGetLastError();
 }
 
-template <int ST>
-hipError_t launch_st(const MigArgs& a, bool count_only, hipStream_t stream, int wpb) {
+template <int ST, int MODE>
+hipError_t launch_st(const MigArgs& a, hipStream_t stream, int wpb) {
   switch (wpb) {
-    case 1: return launch_wpb<ST, 1>(a, count_only, stream);
-    case 2: return launch_wpb<ST, 2>(a, count_only, stream);
-    case 8: return launch_wpb<ST, 8>(a, count_only, stream);
-    case 16: return launch_wpb<ST, 16>(a, count_only, stream);
-    default: return launch_wpb<ST, 4>(a, count_only, stream);
+    case 1: return launch_wpb<ST, 1, MODE>(a, stream);
+    case 2: return launch_wpb<ST, 2, MODE>(a, stream);
+    case 8: return launch_wpb<ST, 8, MODE>(a, stream);
+    case 16: return launch_wpb<ST, 16, MODE>(a, stream);
+    default: return launch_wpb<ST, 4, MODE>(a, stream);
   }
 }
 
-hipError_t launch_migrate(const MigArgs& a0, bool count_only, hipStream_t stream,
-                          const Tuning& tune, int row_extent) {
+hipError_t launch_migrate(const MigArgs& a0, int mode, hipStream_t stream, const Tuning& tune,
+                          int row_extent) {
   if (a0.n_out_il <= 0 || a0.n_out_xl <= 0 || a0.n_t <= 0) return hipSuccess;
   MigArgs a = a0;
   // aperture-row passes: pass p sums the row offsets [p*R, (p+1)*R) (generic
   // rows: row indices); passes after the first accumulate into `out`
-  const int R = (count_only || tune.rows_per_pass <= 0) ? (row_extent + 1) : tune.rows_per_pass;
+  const int R = (mode == kCount || tune.rows_per_pass <= 0) ? (row_extent + 1) : tune.rows_per_pass;
+  const int wpb = tune.waves_per_block;
   for (int lo = 0, pass = 0; lo <= row_extent; lo += R, ++pass) {
     a.d_lo = lo;
     a.d_hi = lo + R;
     a.accumulate = pass > 0 ? 1 : 0;
     hipError_t e;
-    if (count_only) {
-      e = launch_st<PSM_STORE_F32>(a, true, stream, tune.waves_per_block);
-    } else {
-      switch (a.storage) {
-        case PSM_STORE_F16: e = launch_st<PSM_STORE_F16>(a, false, stream, tune.waves_per_block); break;
-        case PSM_STORE_BF16: e = launch_st<PSM_STORE_BF16>(a, false, stream, tune.waves_per_block); break;
-        default: e = launch_st<PSM_STORE_F32>(a, false, stream, tune.waves_per_block); break;
-      }
+    switch (mode) {
+      case kCount: e = launch_st<PSM_STORE_F32, kCount>(a, stream, wpb); break;
+      case kAdjoint: e = launch_st<PSM_STORE_F32, kAdjoint>(a, stream, wpb); break;
+      case kTangent: e = launch_st<PSM_STORE_F32, kTangent>(a, stream, wpb); break;
+      default:
+        switch (a.storage) {
+          case PSM_STORE_F16: e = launch_st<PSM_STORE_F16, kForward>(a, stream, wpb); break;
+          case PSM_STORE_BF16: e = launch_st<PSM_STORE_BF16, kForward>(a, stream, wpb); break;
+          default: e = launch_st<PSM_STORE_F32, kForward>(a, stream, wpb); break;
+        }
     }
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+hipError_t launch_dot_f64(const float* x, const float* y, size_t n, double* partial,
+                          double* out, hipStream_t stream) {
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(kDotBlocks), dim3(256), 0, stream, x, y, n, partial);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(256), 0, stream, partial, kDotBlocks, out);
+  return hipGetLastError();
+}
+
+size_t dot_scratch_doubles() { return kDotBlocks + 1; }
+
+hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n);
+  return hipGetLastError();
 }
 
 }  // namespace psm

@@ -202,6 +202,65 @@ def migrate_volume(data, vel, params: SurveyParams, out=None, ctx: _lib.Context 
     return out
 
 
+def _volume_call(fn_name, inp, vel, params, out, out_shape, ctx, sync):
+    ctx = ctx or _lib.context()
+    ip, iw = _ptr_where(inp, "input")
+    vp, vw = _ptr_where(vel, "vel")
+    if iw != vw:
+        raise PsmError("input and vel must both be host or both be device memory")
+    if tuple(vel.shape) != (params.n_il, params.n_xl, params.n_t):
+        raise PsmError(f"vel shape {tuple(vel.shape)} does not match the survey")
+    if out is None:
+        if _is_torch(inp):
+            import torch
+            out = torch.empty(out_shape, dtype=torch.float32, device=inp.device)
+        else:
+            out = np.empty(out_shape, np.float32)
+    op, ow = _ptr_where(out, "out")
+    if ow != iw:
+        raise PsmError("out must live where the input lives")
+    s = params.to_c()
+    ctx.check(getattr(ctx.lib, fn_name)(ctx.h, C.byref(s), ip, vp, op, iw), fn_name)
+    if sync and iw == PSM_DEVICE:
+        ctx.synchronize()
+    return out
+
+
+def demigrate_volume(image, vel, params: SurveyParams, out=None,
+                     ctx: _lib.Context | None = None, sync: bool = True):
+    """Adjoint of migrate_volume: image [n_out_il][n_out_xl][n_t] -> data
+    [n_il][n_xl][n_t] (Kirchhoff demigration with the forward operator's
+    sample index, fraction, weight and mask)."""
+    info = survey_plan(params, vel)
+    if tuple(image.shape) != (info.n_out_il, info.n_out_xl, info.n_t):
+        raise PsmError(f"image shape {tuple(image.shape)} does not match the survey output "
+                       f"({info.n_out_il}, {info.n_out_xl}, {info.n_t})")
+    return _volume_call("psm_demigrate_volume", image, vel, params, out,
+                        (params.n_il, params.n_xl, params.n_t), ctx, sync)
+
+
+def migrate_volume_dalpha(data, vel, params: SurveyParams, out=None,
+                          ctx: _lib.Context | None = None, sync: bool = True):
+    """d(migrate_volume)/d(alpha) at the context's velocity scale, mask held
+    at its mask scale."""
+    info = survey_plan(params, vel)
+    return _volume_call("psm_migrate_volume_dalpha", data, vel, params, out,
+                        (info.n_out_il, info.n_out_xl, info.n_t), ctx, sync)
+
+
+def dot_f64(x, y, ctx: _lib.Context | None = None) -> float:
+    """sum x*y of two float32 arrays accumulated in float64 on the GPU."""
+    ctx = ctx or _lib.context()
+    xp, xw = _ptr_where(x, "x")
+    yp, yw = _ptr_where(y, "y")
+    n = int(np.prod(x.shape))
+    if xw != yw or n != int(np.prod(y.shape)):
+        raise PsmError("x and y must have the same size and live in the same memory")
+    r = C.c_double()
+    ctx.check(ctx.lib.psm_dot_f64(ctx.h, xp, yp, n, xw, C.byref(r)), "psm_dot_f64")
+    return r.value
+
+
 def count_pairs(vel, params: SurveyParams, ctx: _lib.Context | None = None, images=False):
     """(P_ap, P_g) of the work unit for this survey, counted on the GPU.  With
     images=True also returns the per-output-sample counts (float32 arrays of

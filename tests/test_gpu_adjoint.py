@@ -1,0 +1,151 @@
+"""Adjoint (demigration), velocity-scale tangent, autograd harness and
+CG least-squares migration on the GPU.
+
+Bars: dot-product test |<Mx,y> - <x,M^T y>| / (||Mx|| ||y||) <= 1e-6;
+adjoint against the FP64 oracle adjoint <= 1e-5 rel-L2; the alpha-gradient
+against a central finite difference (mask held fixed) within 1e-3 relative;
+CGLS residual norms non-increasing, and a run resumed from a checkpoint
+equal to an uninterrupted run to 1e-5.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from _cases import rel_l2
+
+import poststackmig3d_amd as P
+from poststackmig3d_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dot_test(data_shape, vel, prm, seed, ctx=None):
+    torch = _torch()
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    info = P.survey_plan(prm, vel)
+    x = torch.randn(data_shape, generator=g, device="cuda")
+    y = torch.randn((info.n_out_il, info.n_out_xl, info.n_t), generator=g, device="cuda")
+    mx = P.migrate_volume(x, vel, prm, ctx=ctx)
+    mty = P.demigrate_volume(y, vel, prm, ctx=ctx)
+    lhs = P.dot_f64(mx, y, ctx=ctx)
+    rhs = P.dot_f64(x, mty, ctx=ctx)
+    norm = np.sqrt(P.dot_f64(mx, mx, ctx=ctx)) * np.sqrt(P.dot_f64(y, y, ctx=ctx))
+    return abs(lhs - rhs) / norm, lhs, rhs
+
+
+@pytest.mark.parametrize("amp,angle,pad", [(True, 45.0, (3, 4)), (False, 30.0, (0, 0)),
+                                           (True, 60.0, (2, 0))])
+def test_dot_product_small(gpu_ctx, amp, angle, pad):
+    torch = _torch()
+    n = (31, 29, 150)
+    vel = torch.from_numpy(synth.production_velocity(*n)).cuda()
+    prm = P.SurveyParams.for_cube(n, inline_padding=pad[0], crossline_padding=pad[1],
+                                  angle_aperture=angle, amp_correction=amp)
+    err, lhs, rhs = _dot_test(n, vel, prm, 1)
+    print(f"dot test amp={amp} angle={angle} pad={pad}: {err:.2e} ({lhs:.6e} vs {rhs:.6e})")
+    assert err <= 1e-6
+
+
+def test_adjoint_matches_fp64_oracle(gpu_ctx):
+    n = (13, 12, 90)
+    vel = synth.production_velocity(*n)
+    prm = P.SurveyParams.for_cube(n, inline_padding=2, crossline_padding=3, angle_aperture=45.0)
+    info = P.survey_plan(prm, vel)
+    img = np.random.default_rng(3).normal(size=(info.n_out_il, info.n_out_xl, info.n_t))
+    img = img.astype(np.float32)
+    got = P.demigrate_volume(img, vel, prm)
+    ref = O.adjoint_volume_f64(img, vel, 0.002, 25.0, 25.0, 2, 3, 45.0, True)
+    assert rel_l2(got, ref) <= 1e-5
+
+
+def test_dot_product_lsm_size(gpu_ctx):
+    """The least-squares case: 201 x 201 x 751, V = 2200 m/s, 30 deg."""
+    torch = _torch()
+    n = (201, 201, 751)
+    vel = torch.full(n, 2200.0, device="cuda")
+    prm = P.SurveyParams.for_cube(n, angle_aperture=30.0)
+    err, lhs, rhs = _dot_test(n, vel, prm, 2)
+    print(f"dot test 201x201x751: {err:.2e}")
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("alpha", [0.95, 1.0, 1.05])
+def test_alpha_gradient_matches_finite_difference(alpha):
+    """J(alpha) = <M_alpha x, y> with the mask at 1: dJ/dalpha from the
+    tangent kernel (and through torch.autograd) against a central difference.
+    The data are smooth (3-8 Hz at 2 ms, tapered to zero before the last
+    samples) so that the linear interpolation's slope changes little between
+    samples and no pair crosses the strict bound i < n_t - 1 within the step:
+    the difference quotient then converges to the derivative."""
+    torch = _torch()
+    from poststackmig3d_amd._lib import Context
+    from poststackmig3d_amd.autograd import migrate
+    n = (31, 27, 160)
+    rng = np.random.default_rng(8)
+    t = np.arange(n[2]) * 0.002
+    taper = np.clip((0.8 * t[-1] - t) / (0.1 * t[-1]), 0.0, 1.0)
+    fr = rng.uniform(3.0, 8.0, size=n[:2])[..., None]
+    ph = rng.uniform(0, 2 * np.pi, size=n[:2])[..., None]
+    x = torch.from_numpy((np.sin(2 * np.pi * fr * t + ph) * taper).astype(np.float32)).cuda()
+    vel = torch.full(n, 2200.0, device="cuda")
+    prm = P.SurveyParams.for_cube(n, angle_aperture=30.0)
+    info = P.survey_plan(prm, vel)
+    y = torch.randn((info.n_out_il, info.n_out_xl, info.n_t), device="cuda",
+                    generator=torch.Generator(device="cuda").manual_seed(5))
+    ctx = Context(0)
+    try:
+        def J(a):
+            ctx.set_velocity_scale(a, 1.0)
+            return P.dot_f64(P.migrate_volume(x, vel, prm, ctx=ctx), y, ctx=ctx)
+        h = 1e-4
+        fd = (J(alpha + h) - J(alpha - h)) / (2 * h)
+        ctx.set_velocity_scale(alpha, 1.0)
+        tan = P.dot_f64(P.migrate_volume_dalpha(x, vel, prm, ctx=ctx), y, ctx=ctx)
+        ctx.set_velocity_scale(1.0, 1.0)
+        a = torch.tensor(alpha, dtype=torch.float64, requires_grad=True)
+        xr = x.clone().requires_grad_(True)
+        out = migrate(xr, a, vel, prm, ctx=ctx)
+        (out * y).sum().backward()
+        # the data gradient through autograd is the adjoint (at alpha) applied to y
+        ctx.set_velocity_scale(alpha, 1.0)
+        ref = P.demigrate_volume(y, vel, prm, ctx=ctx)
+        ctx.set_velocity_scale(1.0, 1.0)
+    finally:
+        ctx.close()
+    print(f"alpha={alpha}: FD {fd:.6e} tangent {tan:.6e} autograd {a.grad.item():.6e}")
+    assert abs(tan - fd) <= 1e-3 * abs(fd)
+    assert abs(a.grad.item() - tan) <= 1e-9 * abs(tan)
+    assert rel_l2(xr.grad.cpu().numpy(), ref.cpu().numpy()) <= 1e-5
+
+
+def test_cgls_decreases_residual_and_resumes(tmp_path):
+    torch = _torch()
+    from poststackmig3d_amd.lsm import cgls
+    n = (61, 61, 251)
+    vel = torch.full(n, 2200.0, device="cuda")
+    prm = P.SurveyParams.for_cube(n, angle_aperture=30.0)
+    m_true = torch.from_numpy(synth.spikes(n, density=0.005, seed=4)).cuda()
+    d = P.demigrate_volume(m_true, vel, prm)
+    g = torch.Generator(device="cuda").manual_seed(6)
+    d = d + 0.05 * d.std() * torch.randn(d.shape, device="cuda", generator=g)
+    full = cgls(d, vel, prm, iterations=6)
+    rn = full.residual_norms
+    print("CGLS residual norms:", ["%.4e" % v for v in rn])
+    assert all(b <= a * (1 + 1e-6) for a, b in zip(rn, rn[1:]))
+    assert rn[-1] < 0.6 * rn[0]
+    # correlation with the true reflectivity improves over plain migration
+    mig = P.migrate_volume(d, vel, prm)
+    corr = lambda u: float((u * m_true).sum() / (u.norm() * m_true.norm()))  # noqa: E731
+    assert corr(full.model) > corr(mig)
+    ck = str(tmp_path / "cg.pt")
+    part = cgls(d, vel, prm, iterations=6, checkpoint=ck, stop_after=3)
+    assert part.iterations == 3
+    resumed = cgls(d, vel, prm, iterations=6, resume=ck)
+    assert resumed.iterations == 6
+    assert rel_l2(resumed.model.cpu().numpy(), full.model.cpu().numpy()) <= 1e-5
+    np.testing.assert_allclose(resumed.residual_norms, rn, rtol=1e-5)
