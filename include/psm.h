@@ -167,6 +167,37 @@ int psm_migrate_volume_dalpha(psm_ctx* ctx, const psm_survey* survey,
                               const float* data, const float* vel, float* out,
                               int where);
 
+/*
+ * Batched migration (no reference counterpart): several volumes -- e.g. a
+ * velocity scan over scaled Vrms models and sub-volumes of any size -- in one
+ * persistent launch.  Every workgroup drains a shared queue of
+ * (volume, t0 tile, crossline group) items, latest tiles of all volumes
+ * first, so uneven volumes balance.  psm_batch_create packs nothing yet: it
+ * plans every item and captures one HIP graph [queue reset, packing of each
+ * distinct data cube, persistent kernel]; psm_batch_run replays it on the
+ * context's stream (ms != NULL: waits and returns the elapsed time).
+ * All item pointers are device memory and must stay valid while the batch
+ * lives.  Item k's output equals psm_migrate_volume of that item at velocity
+ * scale (alpha, alpha).  psm_batch_autotune times candidate launch shapes
+ * (workgroup width x resident workgroups per CU) on the batch itself and keeps
+ * the fastest.  Storage: PSM_STORE_* for all items.
+ */
+typedef struct psm_batch psm_batch;
+typedef struct psm_batch_item {
+    psm_survey survey;  /* geometry and parameters of this volume */
+    const float* data;  /* device [n_il][n_xl][n_t] */
+    const float* vel;   /* device [n_il][n_xl][n_t] */
+    float* out;         /* device [n_out_il][n_out_xl][n_t] */
+    float alpha;        /* velocity scale of this volume (1 = vel as given) */
+} psm_batch_item;
+int psm_batch_create(psm_ctx* ctx, const psm_batch_item* items, int n_items, int storage,
+                     psm_batch** batch);
+int psm_batch_set_shape(psm_batch* batch, int waves_per_block, int blocks_per_cu);
+int psm_batch_autotune(psm_batch* batch, int* waves_per_block, int* blocks_per_cu,
+                       float* best_ms);
+int psm_batch_run(psm_batch* batch, float* ms);
+void psm_batch_destroy(psm_batch* batch);
+
 /* sum x[i] y[i] accumulated in float64 on the GPU (deterministic order);
  * x, y host or device per `where`. */
 int psm_dot_f64(psm_ctx* ctx, const float* x, const float* y, size_t n, int where,

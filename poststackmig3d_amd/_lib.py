@@ -44,6 +44,12 @@ class Plan(C.Structure):
     ]
 
 
+class BatchItem(C.Structure):
+    """psm_batch_item (include/psm.h)."""
+    _fields_ = [("survey", Survey), ("data", C.c_void_p), ("vel", C.c_void_p),
+                ("out", C.c_void_p), ("alpha", C.c_float)]
+
+
 class Timing(C.Structure):
     """psm_timing (include/psm.h)."""
     _fields_ = [("pack_ms", C.c_float), ("migrate_ms", C.c_float)]
@@ -82,6 +88,13 @@ SIGNATURES = [
                                             C.c_void_p, C.c_void_p, C.c_int]),
     ("psm_dot_f64", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
                               C.POINTER(C.c_double)]),
+    ("psm_batch_create", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                   C.POINTER(C.c_void_p)]),
+    ("psm_batch_set_shape", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("psm_batch_autotune", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_float)]),
+    ("psm_batch_run", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    ("psm_batch_destroy", None, [C.c_void_p]),
     ("psm_set_storage", C.c_int, [C.c_void_p, C.c_int]),
     ("psm_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("psm_get_tuning", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

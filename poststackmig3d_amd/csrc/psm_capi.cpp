@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -657,6 +658,286 @@ int psm_last_timing(psm_ctx* c, psm_timing* t) {
   PSM_HIP(c, hipEventElapsedTime(&t->pack_ms, c->ev[0], c->ev[1]));
   PSM_HIP(c, hipEventElapsedTime(&t->migrate_ms, c->ev[1], c->ev[2]));
   return PSM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+// Several volumes (a velocity scan over scaled Vrms models, sub-volumes of
+// any size) migrated by one persistent launch, captured once into a HIP graph
+// together with the trace packing and the queue reset, then replayed.
+
+struct psm_batch {
+  psm_ctx* ctx = nullptr;
+  int storage = PSM_STORE_F32;
+  int n_items = 0;
+  int wpb = 4, blocks_per_cu = 0;
+  int n_cu = 0;
+  std::vector<psm::MigArgs> args;
+  std::vector<psm::BatchSeg> segs;
+  unsigned total = 0;
+  // per distinct data cube: raw pointer, shape, packed buffer, scales
+  struct Group {
+    const float* data;
+    int n_il, n_xl, n_t;
+    DevBuf pairs, scale;
+  };
+  std::vector<std::unique_ptr<Group>> groups;
+  std::vector<std::unique_ptr<DevBuf>> item_bufs;  // tasks and t0 per item
+  DevBuf d_args, d_segs, counter;
+  std::vector<int> n_tiles;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+void batch_release_graph(psm_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Captures [queue reset, packing of every data cube, persistent kernel].
+int batch_capture(psm_batch* b) {
+  psm_ctx* c = b->ctx;
+  batch_release_graph(b);
+  hipStream_t st = c->own_stream;  // capture needs a stream of our own
+  const int blocks = b->n_cu * (b->blocks_per_cu > 0 ? b->blocks_per_cu : std::max(1, 32 / b->wpb));
+  PSM_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  hipError_t e = psm::launch_fill_zero(b->counter.as<float>(), 1, st);
+  for (auto& g : b->groups) {
+    if (e != hipSuccess) break;
+    e = psm::launch_pack(b->storage, g->data, g->pairs.p, g->scale.as<float>(), g->n_il, g->n_xl,
+                         g->n_t, st);
+  }
+  if (e == hipSuccess)
+    e = psm::launch_batch(b->storage, b->d_args.as<psm::MigArgs>(), b->d_segs.as<psm::BatchSeg>(),
+                          (int)b->segs.size(), b->counter.as<unsigned>(), b->total, b->wpb, blocks,
+                          st);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(st, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(c, PSM_ERR_HIP, "batch graph capture failed: %s",
+                hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  b->graph = g;
+  PSM_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return PSM_OK;
+}
+
+// Builds the work queue for workgroups of wpb waves (latest t0 tiles of all
+// volumes first), uploads it and captures the graph.
+int batch_set_shape(psm_batch* b, int wpb, int blocks_per_cu) {
+  psm_ctx* c = b->ctx;
+  if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8)
+    return fail(c, PSM_ERR_ARG, "batch waves_per_block must be 1, 2, 4 or 8");
+  if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(c, PSM_ERR_ARG, "bad blocks_per_cu");
+  b->wpb = wpb;
+  b->blocks_per_cu = blocks_per_cu;
+  b->segs.clear();
+  int max_tiles = 0;
+  for (int t : b->n_tiles) max_tiles = std::max(max_tiles, t);
+  long long first = 0;
+  for (int r = 0; r < max_tiles; ++r)
+    for (int k = 0; k < b->n_items; ++k) {
+      if (b->n_tiles[k] <= r) continue;
+      const psm::MigArgs& a = b->args[k];
+      const int njg = (a.n_out_xl + wpb - 1) / wpb;
+      const long long cnt = (long long)a.n_out_il * njg;
+      if (first + cnt > 0x7fffffffLL) return fail(c, PSM_ERR_ARG, "batch queue too long");
+      b->segs.push_back(psm::BatchSeg{k, b->n_tiles[k] - 1 - r, (int)first, (int)cnt, njg});
+      first += cnt;
+    }
+  b->total = (unsigned)first;
+  hipStream_t st = c->own_stream;
+  PSM_HIP(c, upload(b->d_args, b->args.data(), b->args.size() * sizeof(psm::MigArgs), st));
+  PSM_HIP(c, upload(b->d_segs, b->segs.data(), b->segs.size() * sizeof(psm::BatchSeg), st));
+  PSM_HIP(c, hipStreamSynchronize(st));
+  return batch_capture(b);
+}
+
+int batch_run_locked(psm_batch* b, float* ms) {
+  psm_ctx* c = b->ctx;
+  PSM_HIP(c, hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  if (ms) {
+    if (!b->ev0) PSM_HIP(c, hipEventCreate(&b->ev0));
+    if (!b->ev1) PSM_HIP(c, hipEventCreate(&b->ev1));
+    PSM_HIP(c, hipEventRecord(b->ev0, st));
+  }
+  PSM_HIP(c, hipGraphLaunch(b->exec, st));
+  if (ms) {
+    PSM_HIP(c, hipEventRecord(b->ev1, st));
+    PSM_HIP(c, hipEventSynchronize(b->ev1));
+    PSM_HIP(c, hipEventElapsedTime(ms, b->ev0, b->ev1));
+  }
+  return PSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psm_batch_create(psm_ctx* c, const psm_batch_item* items, int n_items, int storage,
+                     psm_batch** out) {
+  if (!c || !out || !items || n_items <= 0) return c ? fail(c, PSM_ERR_ARG, "bad batch arguments") : PSM_ERR_ARG;
+  if (storage != PSM_STORE_F32 && storage != PSM_STORE_F16 && storage != PSM_STORE_BF16)
+    return fail(c, PSM_ERR_ARG, "unknown storage %d", storage);
+  std::lock_guard<std::mutex> lk(c->mu);
+  PSM_HIP(c, hipSetDevice(c->device));
+  std::unique_ptr<psm_batch> b(new (std::nothrow) psm_batch());
+  if (!b) return fail(c, PSM_ERR_NOMEM, "host allocation failed");
+  b->ctx = c;
+  b->storage = storage;
+  b->n_items = n_items;
+  PSM_HIP(c, hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, c->device));
+  hipStream_t st = c->own_stream;
+  for (int k = 0; k < n_items; ++k) {
+    const psm_batch_item& it = items[k];
+    const psm_survey* s = &it.survey;
+    if (!it.data || !it.vel || !it.out) return fail(c, PSM_ERR_ARG, "batch item %d: null pointer", k);
+    if (!(it.alpha > 0)) return fail(c, PSM_ERR_ARG, "batch item %d: alpha must be > 0", k);
+    psm_plan plan;
+    int rc = plan_impl(c, s, it.vel, PSM_DEVICE, &plan, it.alpha);
+    if (rc) return rc;
+    struct PlanGuard {
+      psm_plan* p;
+      ~PlanGuard() { psm_plan_free(p); }
+    } guard{&plan};
+    if (((size_t)s->n_xl + 1) * s->n_t * sizeof(float2) >= (size_t)INT32_MAX)
+      return fail(c, PSM_ERR_ARG, "batch item %d: one inline exceeds 2 GiB", k);
+    // data cube group (items migrating the same cube share its packing)
+    psm_batch::Group* grp = nullptr;
+    for (auto& g : b->groups)
+      if (g->data == it.data && g->n_il == (int)s->n_il && g->n_xl == (int)s->n_xl &&
+          g->n_t == (int)s->n_t)
+        grp = g.get();
+    if (!grp) {
+      b->groups.emplace_back(new psm_batch::Group{it.data, (int)s->n_il, (int)s->n_xl, (int)s->n_t, {}, {}});
+      grp = b->groups.back().get();
+      PSM_ALLOC(c, grp->pairs, psm::pairs_bytes(storage, grp->n_il, grp->n_xl, grp->n_t));
+      PSM_ALLOC(c, grp->scale, (size_t)(grp->n_il + 1) * (grp->n_xl + 1) * sizeof(float));
+    }
+    std::vector<psm::Task> tasks(plan.n_out_il);
+    for (size_t io = 0; io < plan.n_out_il; ++io)
+      tasks[io] = psm::Task{plan.out_to_in_il[io], plan.window[2 * io], plan.window[2 * io + 1],
+                            plan.out_to_in_il[io]};
+    b->item_bufs.emplace_back(new DevBuf());
+    DevBuf& tb = *b->item_bufs.back();
+    b->item_bufs.emplace_back(new DevBuf());
+    DevBuf& t0b = *b->item_bufs.back();
+    PSM_HIP(c, upload(tb, tasks.data(), tasks.size() * sizeof(psm::Task), st));
+    PSM_HIP(c, upload(t0b, plan.t0_times, s->n_t * sizeof(float), st));
+    psm::MigArgs a{};
+    a.xp = grp->pairs.p;
+    a.scale = grp->scale.as<float>();
+    a.n_rows = (int)s->n_il;
+    a.storage = storage;
+    a.vel = it.vel;
+    a.out = it.out;
+    a.t0 = t0b.as<float>();
+    a.tasks = tb.as<psm::Task>();
+    a.n_t = (int)s->n_t;
+    a.n_xd = (int)s->n_xl;
+    a.xd_off = s->crossline_padding;
+    a.n_xv = (int)s->n_xl;
+    a.xv_off = s->crossline_padding;
+    a.K = (int)(s->n_xl + 2 * (size_t)s->crossline_padding);
+    a.n_out_xl = (int)plan.n_out_xl;
+    a.n_out_il = (int)plan.n_out_il;
+    const float dtf = (float)s->dt;
+    a.inv_dt = 1.0f / dtf;
+    a.dx = (float)s->inline_step;
+    a.dy = (float)s->crossline_step;
+    a.inv_dy = 1.0f / a.dy;
+    a.tan2 = plan.tan_theta_sq;
+    a.amp = s->amp_correction ? 1 : 0;
+    a.d_lo = 0;
+    a.d_hi = 1 << 30;
+    a.alpha = it.alpha;
+    a.alpha_mask = it.alpha;
+    b->args.push_back(a);
+    b->n_tiles.push_back((int)((s->n_t + 63) / 64));
+  }
+  PSM_ALLOC(c, b->counter, 256);
+  PSM_HIP(c, hipStreamSynchronize(st));
+  // the queue and the graph depend on the workgroup width
+  const int rc = batch_set_shape(b.get(), 4, 0);
+  if (rc) {
+    batch_release_graph(b.get());
+    return rc;
+  }
+  *out = b.release();
+  return PSM_OK;
+}
+
+int psm_batch_set_shape(psm_batch* b, int waves_per_block, int blocks_per_cu) {
+  if (!b) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  PSM_HIP(b->ctx, hipSetDevice(b->ctx->device));
+  return batch_set_shape(b, waves_per_block, blocks_per_cu);
+}
+
+int psm_batch_run(psm_batch* b, float* ms) {
+  if (!b) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  return batch_run_locked(b, ms);
+}
+
+int psm_batch_autotune(psm_batch* b, int* waves_per_block, int* blocks_per_cu, float* best_ms) {
+  if (!b) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  psm_ctx* c = b->ctx;
+  PSM_HIP(c, hipSetDevice(c->device));
+  // candidate launch shapes: workgroup width x resident workgroups per CU
+  const int cand[][2] = {{1, 32}, {2, 16}, {4, 8}, {8, 4}, {2, 8}, {4, 4}};
+  float best = -1.0f;
+  int bw = 4, bb = 0;
+  for (const auto& cd : cand) {
+    int rc = batch_set_shape(b, cd[0], cd[1]);
+    if (rc) return rc;
+    float ms = 0.0f;
+    rc = batch_run_locked(b, nullptr);  // warm-up
+    if (rc) return rc;
+    rc = batch_run_locked(b, &ms);
+    if (rc) return rc;
+    if (best < 0.0f || ms < best) {
+      best = ms;
+      bw = cd[0];
+      bb = cd[1];
+    }
+  }
+  const int rc = batch_set_shape(b, bw, bb);
+  if (rc) return rc;
+  if (waves_per_block) *waves_per_block = bw;
+  if (blocks_per_cu) *blocks_per_cu = bb;
+  if (best_ms) *best_ms = best;
+  return PSM_OK;
+}
+
+void psm_batch_destroy(psm_batch* b) {
+  if (!b) return;
+  psm_ctx* c = b->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->own_stream);
+  batch_release_graph(b);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  for (auto& g : b->groups) {
+    g->pairs.release();
+    g->scale.release();
+  }
+  for (auto& d : b->item_bufs) d->release();
+  b->d_args.release();
+  b->d_segs.release();
+  b->counter.release();
+  delete b;
 }
 
 }  // extern "C"

@@ -56,6 +56,12 @@ struct MigArgs {
   float* adj_out;          // adjoint: data [n_rows][n_xd][n_t], accumulated (zeroed by the caller)
 };
 
+// One segment of a batch's work queue: the crossline groups of one t0 tile of
+// one volume, occupying queue positions [first, first + count).
+struct BatchSeg {
+  int item, tile, first, count, n_jgroups;
+};
+
 // Launch-shape parameters (chosen by the autotuner or the caller).
 struct Tuning {
   int waves_per_block = 4;   // output crosslines per workgroup (1, 2, 4, 8, 16)
@@ -76,6 +82,10 @@ hipError_t launch_pack(int storage, const float* x, void* xp, float* scale, int 
 hipError_t launch_migrate(const MigArgs& a, int mode, hipStream_t stream, const Tuning& tune,
                           int row_extent);
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream);
+// Persistent batch launch (forward migration of several volumes): `blocks`
+// workgroups of wpb waves drain `total` queue positions described by segs.
+hipError_t launch_batch(int storage, const MigArgs* items, const BatchSeg* segs, int n_segs,
+                        unsigned* counter, unsigned total, int wpb, int blocks, hipStream_t stream);
 // out[0] = sum x[i] y[i] in float64; partial holds dot_scratch_doubles() - 1 doubles.
 size_t dot_scratch_doubles();
 hipError_t launch_dot_f64(const float* x, const float* y, size_t n, double* partial, double* out,

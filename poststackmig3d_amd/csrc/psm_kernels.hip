@@ -388,20 +388,12 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
   }
 }
 
-template <int ST, int WPB, bool GENERIC_ROWS, int MODE>
-__global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_tiles,
-                                                              int n_jgroups, int per_tile_pad) {
+// The work of one wave: output trace (io, j) of the output inline io, t0 tile
+// `tile` (64 samples).
+template <int ST, bool GENERIC_ROWS, int MODE>
+__device__ __forceinline__ void migrate_trace_tile(const MigArgs& a, int tile, int io, int j) {
   constexpr bool COUNT = MODE == kCount;
   constexpr bool WRITES_OUT = MODE == kForward || MODE == kTangent;
-  // ---- block -> (t0 tile, output inline, crossline group), heavy tiles first
-  const int g = blockIdx.x;
-  const int tile = n_tiles - 1 - g / per_tile_pad;
-  const int rem = g % per_tile_pad;
-  const int spatial = (rem % kXcds) * (per_tile_pad / kXcds) + rem / kXcds;
-  if (spatial >= a.n_out_il * n_jgroups) return;
-  const int io = spatial / n_jgroups;
-  const int j = (spatial % n_jgroups) * WPB + uniform_i((int)(threadIdx.x / kWave));
-  if (j >= a.n_out_xl) return;
   const int lane = threadIdx.x % kWave;
   const int s = tile * kWave + lane;
   const bool active = s < a.n_t;
@@ -478,6 +470,54 @@ __global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_t
     } else {
       orow[s] = (src >= 0) ? acc : 0.0f;  // :148-151, :375
     }
+  }
+}
+
+template <int ST, int WPB, bool GENERIC_ROWS, int MODE>
+__global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_tiles,
+                                                              int n_jgroups, int per_tile_pad) {
+  // ---- block -> (t0 tile, output inline, crossline group), heavy tiles first
+  const int g = blockIdx.x;
+  const int tile = n_tiles - 1 - g / per_tile_pad;
+  const int rem = g % per_tile_pad;
+  const int spatial = (rem % kXcds) * (per_tile_pad / kXcds) + rem / kXcds;
+  if (spatial >= a.n_out_il * n_jgroups) return;
+  const int io = spatial / n_jgroups;
+  const int j = (spatial % n_jgroups) * WPB + uniform_i((int)(threadIdx.x / kWave));
+  if (j >= a.n_out_xl) return;
+  migrate_trace_tile<ST, GENERIC_ROWS, MODE>(a, tile, io, j);
+}
+
+// Persistent batch kernel: every workgroup takes work items (one
+// (volume, t0 tile, crossline group) each) from a global counter until the
+// queue is drained, so volumes of very different size and cost share one
+// launch.  Items are ordered by segment: segment k = the groups of one t0
+// tile of one volume, latest tiles of all volumes first.  Each workgroup
+// exits once the counter passes `total`.
+template <int ST, int WPB>
+__global__ __launch_bounds__(kWave * WPB) void batch_kernel(const MigArgs* __restrict__ items,
+                                                            const BatchSeg* __restrict__ segs,
+                                                            int n_segs,
+                                                            unsigned* __restrict__ counter,
+                                                            unsigned total) {
+  __shared__ unsigned s_next;
+  for (;;) {
+    if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
+    __syncthreads();
+    const unsigned w = s_next;
+    __syncthreads();
+    if (w >= total) return;
+    int lo = 0, hi = n_segs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if ((unsigned)segs[mid].first <= w) lo = mid; else hi = mid - 1;
+    }
+    const BatchSeg sg = segs[lo];
+    const int spatial = (int)w - sg.first;
+    const MigArgs& a = items[sg.item];
+    const int io = spatial / sg.n_jgroups;
+    const int j = (spatial % sg.n_jgroups) * WPB + uniform_i((int)(threadIdx.x / kWave));
+    if (io < a.n_out_il && j < a.n_out_xl) migrate_trace_tile<ST, false, kForward>(a, sg.tile, io, j);
   }
 }
 
@@ -670,6 +710,28 @@ hipError_t launch_dot_f64(const float* x, const float* y, size_t n, double* part
 }
 
 size_t dot_scratch_doubles() { return kDotBlocks + 1; }
+
+template <int ST>
+hipError_t launch_batch_st(const MigArgs* items, const BatchSeg* segs, int n_segs, unsigned* counter,
+                           unsigned total, int wpb, int blocks, hipStream_t stream) {
+  const dim3 grid((unsigned)blocks);
+  switch (wpb) {
+    case 1: hipLaunchKernelGGL((batch_kernel<ST, 1>), grid, dim3(kWave), 0, stream, items, segs, n_segs, counter, total); break;
+    case 2: hipLaunchKernelGGL((batch_kernel<ST, 2>), grid, dim3(2 * kWave), 0, stream, items, segs, n_segs, counter, total); break;
+    case 8: hipLaunchKernelGGL((batch_kernel<ST, 8>), grid, dim3(8 * kWave), 0, stream, items, segs, n_segs, counter, total); break;
+    default: hipLaunchKernelGGL((batch_kernel<ST, 4>), grid, dim3(4 * kWave), 0, stream, items, segs, n_segs, counter, total); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch(int storage, const MigArgs* items, const BatchSeg* segs, int n_segs,
+                        unsigned* counter, unsigned total, int wpb, int blocks, hipStream_t stream) {
+  switch (storage) {
+    case PSM_STORE_F16: return launch_batch_st<PSM_STORE_F16>(items, segs, n_segs, counter, total, wpb, blocks, stream);
+    case PSM_STORE_BF16: return launch_batch_st<PSM_STORE_BF16>(items, segs, n_segs, counter, total, wpb, blocks, stream);
+    default: return launch_batch_st<PSM_STORE_F32>(items, segs, n_segs, counter, total, wpb, blocks, stream);
+  }
+}
 
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
