@@ -17,7 +17,8 @@ stream it runs on) and `cpu_baseline` (the reference's own kernel, compiled
 from its sources into oracle/_ref, on a bounded sample of the same workload
 on this host's cores).
 
-  python bench.py [--steps K] [--warmup W] [--workload production|point]
+  python bench.py [--steps K] [--warmup W] [--workload production|point|scan]
+  python bench.py --workload scan     # batched velocity scan, one graph launch per step
   python bench.py --copy-calibration   # plain 1 GiB device copy, for PMC scaling
 """
 from __future__ import annotations
@@ -40,7 +41,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["production", "point"], default="production")
+    ap.add_argument("--workload", choices=["production", "point", "scan"], default="production")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the FP16/BF16 storage variants of the production line")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-rows", type=int, default=16, help="output inlines in the CPU sample")
     ap.add_argument("--cpu-xl", type=int, default=96, help="output crosslines per sampled inline")
@@ -107,6 +110,104 @@ def cpu_baseline(data_np, vel_np, geo, ap_img, rows_n, m):
                       f"{dt:.2f} s, one inline per OpenMP thread (main.cpp:255 layout)",
             "seconds": dt}
 
+SCAN_ALPHAS = 16
+SCAN_SUBVOLS = 8
+
+
+def scan_items(torch, dev, P, synth):
+    """Velocity scan: the point-diffractor generator (V = 2000 m/s, 25 Hz
+    Ricker at the cube centre, 10 % noise, 30 deg, amp on, no padding) at
+    301 x 301 x 1001, migrated under 16 scales alpha = 0.85 .. 1.15 of its
+    Vrms, plus 8 centred sub-volumes from 64 x 64 x 501 to 301 x 301 x 1001
+    at alpha = 1."""
+    import numpy as np
+    d_np, v_np = synth.point_diffractor(301, 301, 1001)
+    base = torch.from_numpy(d_np).to(dev)
+    vbase = torch.from_numpy(v_np).to(dev)
+    del d_np, v_np
+    geo = dict(dt=0.002, inline_step=25.0, crossline_step=25.0, inline_padding=0,
+               crossline_padding=0, angle_aperture=30.0, amp_correction=True)
+    items = []
+    for alpha in np.linspace(0.85, 1.15, SCAN_ALPHAS):
+        prm = P.SurveyParams.for_cube(base.shape, **geo)
+        items.append((prm, base, vbase, float(alpha)))
+    n = np.linspace(64, 301, SCAN_SUBVOLS).round().astype(int)
+    nt = np.linspace(501, 1001, SCAN_SUBVOLS).round().astype(int)
+    for a, t in zip(n, nt):
+        o = (301 - a) // 2
+        d = base[o:o + a, o:o + a, :t].contiguous()
+        v = vbase[o:o + a, o:o + a, :t].contiguous()
+        items.append((P.SurveyParams.for_cube(d.shape, **geo), d, v, 1.0))
+    return items
+
+
+def run_scan(args, torch, P, synth, dev):
+    from poststackmig3d_amd.batch import Batch
+    ctx = P.context(0)
+    items = scan_items(torch, dev, P, synth)
+    full = []
+    p_ap = 0.0
+    p_g = 0.0
+    for prm, d, v, alpha in items:
+        info = P.survey_plan(prm, v)
+        ctx.set_velocity_scale(alpha, alpha)
+        a, g = P.count_pairs(v, prm, ctx=ctx)
+        p_ap += a
+        p_g += g
+        out = torch.empty((info.n_out_il, info.n_out_xl, info.n_t), dtype=torch.float32,
+                          device=dev)
+        full.append((prm, d, v, out, alpha))
+    ctx.set_velocity_scale(1.0, 1.0)
+    log(f"scan: {len(full)} items, P_ap={p_ap:.6e} P_g={p_g:.6e}")
+    res = {}
+    for storage in ("f32", "f16"):
+        b = Batch(full, storage=storage, ctx=ctx)
+        w, bpc, best = b.autotune()
+        log(f"scan {storage}: autotuned waves/block={w} blocks/CU={bpc} ({best:.1f} ms)")
+        for _ in range(args.warmup):
+            b.run()
+        ms = []
+        torch.cuda.synchronize()
+        t_wall = time.perf_counter()
+        for i in range(args.steps):
+            ms.append(b.run(timed=True))
+            log(f"scan {storage} step {i + 1}/{args.steps}: {ms[-1]:.2f} ms")
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_wall
+        res[storage] = {"value": p_ap / (wall / args.steps), "ms_per_step": wall * 1e3 / args.steps,
+                        "graph_ms": sum(ms) / len(ms), "waves_per_block": w,
+                        "blocks_per_cu": bpc}
+        b.close()
+    f32 = res["f32"]
+    line = {
+        "metric": "migrated sample-aperture-trace pairs/sec (1 GPU)",
+        "value": f32["value"],
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f32["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "velocity scan: point diffractor 301x301x1001 @2ms, V=2000 x "
+                               "16 alpha in [0.85,1.15] + 8 sub-volumes 64x64x501..301x301x1001, "
+                               "30 deg, amp on; one persistent graph-captured launch per step",
+                   "items": len(full), "p_ap": p_ap, "p_g": p_g, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(8.0 * p_g / (f32["graph_ms"] / 1e3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(8.0 * p_g / (f32["graph_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "batch_kernel (graph incl. packs)"},
+        "cpu_baseline": None,
+        "storage_variants": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv)
+                                 for kk, vv in r.items()} for k, r in res.items()},
+        "f16_over_f32": res["f16"]["value"] / f32["value"],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
 
 def main():
     args = parse()
@@ -127,6 +228,8 @@ def main():
     from poststackmig3d_amd import synth
 
     dev = torch.device("cuda:0")
+    if args.workload == "scan":
+        return run_scan(args, torch, P, synth, dev)
     stream = torch.cuda.Stream(device=dev)
     ctx = P.context(0)
     ctx.set_stream(stream.cuda_stream)
@@ -205,6 +308,23 @@ def main():
                         "each trace window from L1/L2 across neighbouring output traces, so "
                         "HBM traffic is far below this and frac > 1 is possible"}
 
+    variants = {}
+    if not args.no_variants:
+        for storage in ("f16", "bf16"):
+            ctx.set_storage(storage)
+            P.migrate_volume(data, vel, prm, out=out, ctx=ctx, sync=True)
+            n = max(1, min(args.steps, 3))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                P.migrate_volume(data, vel, prm, out=out, ctx=ctx, sync=False)
+            torch.cuda.synchronize()
+            w = (time.perf_counter() - t0) / n
+            variants[storage] = {"value": p_ap / w, "ms_per_step": round(w * 1e3, 3),
+                                 "over_f32": round((p_ap / w) / value, 3)}
+            log(f"storage {storage}: {w * 1e3:.1f} ms/step, {variants[storage]['over_f32']:.2f}x f32")
+        ctx.set_storage("f32")
+
     cpu = None
     if not args.no_cpu:
         d_np = data.cpu().numpy()
@@ -233,6 +353,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "storage_variants": variants or None,
         "gpu_over_cpu": (value / cpu["value"]) if cpu and cpu["value"] > 0 else None,
     }
     print(json.dumps(line), flush=True)

@@ -897,12 +897,16 @@ int psm_batch_autotune(psm_batch* b, int* waves_per_block, int* blocks_per_cu, f
   const int cand[][2] = {{1, 32}, {2, 16}, {4, 8}, {8, 4}, {2, 8}, {4, 4}};
   float best = -1.0f;
   int bw = 4, bb = 0;
+  bool warm = false;
   for (const auto& cd : cand) {
     int rc = batch_set_shape(b, cd[0], cd[1]);
     if (rc) return rc;
     float ms = 0.0f;
-    rc = batch_run_locked(b, nullptr);  // warm-up
-    if (rc) return rc;
+    if (!warm) {  // one untimed replay first: code object load, first-touch of the outputs
+      rc = batch_run_locked(b, nullptr);
+      if (rc) return rc;
+      warm = true;
+    }
     rc = batch_run_locked(b, &ms);
     if (rc) return rc;
     if (best < 0.0f || ms < best) {
