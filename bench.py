@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["production", "point", "scan"], default="production")
+    ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
+                    help="trace storage of the main line (compute stays FP32)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP16/BF16 storage variants of the production line")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -258,6 +260,7 @@ def main():
     log(f"workload {desc}; output {info.n_out_il}x{info.n_out_xl}x{info.n_t}, "
         f"ap={info.max_aperture_il}, vmax={info.max_velocity}")
 
+    ctx.set_storage(args.storage)
     p_ap, p_g, ap_img, _ = P.count_pairs(vel, prm, ctx=ctx, images=True)
     log(f"P_ap={p_ap:.6e} P_g={p_g:.6e}")
     out = torch.empty((info.n_out_il, info.n_out_xl, info.n_t), dtype=torch.float32, device=dev)
@@ -289,14 +292,15 @@ def main():
     value = p_ap / (wall / args.steps)
 
     avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
-    alg_bytes = 8.0 * p_g
+    pair_bytes = 8.0 if args.storage == "f32" else 4.0   # one stored (x[i], x[i+1]-x[i]) pair
+    alg_bytes = pair_bytes * p_g
     achieved = alg_bytes / avg_kernel_s / 1e9
     traffic = None
     if os.path.exists(PMC_FILE):
         try:
             with open(PMC_FILE) as f:
                 pmc = json.load(f)
-            if pmc.get("workload") == args.workload:
+            if pmc.get("workload") == args.workload and pmc.get("storage", "f32") == args.storage:
                 traffic = pmc.get("traffic_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
@@ -304,12 +308,12 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "migrate_kernel<false,false>", "kernel_ms": round(avg_kernel_s * 1e3, 3),
                 "algorithmic_bytes": alg_bytes,
-                "note": "algorithmic bytes = 8 B per gathering pair (P_g); the kernel reuses "
+                "note": f"algorithmic bytes = {pair_bytes:.0f} B per gathering pair (P_g); the kernel reuses "
                         "each trace window from L1/L2 across neighbouring output traces, so "
                         "HBM traffic is far below this and frac > 1 is possible"}
 
     variants = {}
-    if not args.no_variants:
+    if not args.no_variants and args.storage == "f32":
         for storage in ("f16", "bf16"):
             ctx.set_storage(storage)
             P.migrate_volume(data, vel, prm, out=out, ctx=ctx, sync=True)
@@ -350,6 +354,7 @@ def main():
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": desc, "n_il": shape[0], "n_xl": shape[1], "n_t": shape[2],
                    "out": [info.n_out_il, info.n_out_xl, info.n_t], "p_ap": p_ap, "p_g": p_g,
+                   "trace_storage": args.storage,
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -259,8 +259,9 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
       // dk is consumed (the scheduling barriers keep the compiler from
       // hoisting the consumption above them).  Steps past kmax are prefetched
       // and consumed unconditionally, so the load counter is the same on
-      // every path; such a step either has r^2 > R2max (mask false in every
-      // lane) or lies beyond the data on both sides (reads the zero trace).
+      // every path; such a step either has r^2 > R2max (every lane masked or
+      // past the end of its trace: a zero term) or lies beyond the data on
+      // both sides (reads the zero trace).
       Step<ST> A, B;
       prep(0, A);
       for (int dk = 0; dk <= kmax; dk += 2) {
@@ -447,7 +448,22 @@ __device__ __forceinline__ void migrate_trace_tile(const MigArgs& a, int tile, i
       needs_window = (p.wlo > lo_need) || (p.whi < hi_need);
     }
   }
-  const float R2max = uniform_f(wave_max(p.R2));
+  // Sweep radius.  Counting needs every pair inside the mask.  The other
+  // modes also stop where the travel time has passed the end of the trace in
+  // every lane: with q = tau/dt >= n_t - 1 a term is zero (:298), and q is
+  // nondecreasing in r^2, so the lane's last useful r^2 is below
+  //   r2_end = ((n_t - 1/2) dt)^2 - t0^2) / a
+  // (half a sample of slack, evaluated in double and rounded up, so no term
+  // that the float arithmetic gathers is cut off).
+  float L = p.R2;
+  if constexpr (MODE != kCount) {
+    if (src >= 0) {
+      const double t_end = ((double)(a.n_t - 1) + 0.5) / (double)a.inv_dt;
+      const double r2_end = (t_end * t_end - (double)p.t0sq) / (double)p.a;
+      L = fminf(L, (float)(r2_end * (1.0 + 1e-6)));
+    }
+  }
+  const float R2max = uniform_f(wave_max(L));
   float acc = 0.0f;
   unsigned cap = 0, cg = 0;
   if (R2max >= 0.0f) {
