@@ -42,6 +42,8 @@
 // -ffp-contract=off so that these FMAs are exactly the ones written here.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "psm_internal.h"
 
 namespace psm {
@@ -145,17 +147,6 @@ struct LaneParams {
   float mval;  // adjoint: the image sample this lane scatters
 };
 
-// One (|db|, |dk|) step of the aperture sum: the fraction, weight and mask
-// shared by the up-to-four traces with the same r^2, their four gathers and
-// (reduced-precision storage) their four trace scales.
-template <int ST>
-struct Step {
-  float f, w;
-  bool m;
-  typename Store<ST>::Raw x0, x1, x2, x3;
-  float s0, s1, s2, s3;
-};
-
 // Sums the aperture of one (output trace, t0 tile) over the row offsets
 // |db| in [d_lo, d_hi) of this pass.  WINDOW = some lane uses parameters of
 // an earlier crossline whose crossline window can cut the aperture (sticky
@@ -202,75 +193,136 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
     if (kmax < 0) return;
     const const_f32_ptr scp = as_const(a.scale + (size_t)ip * row_tr);
     const const_f32_ptr scm = as_const(a.scale + (size_t)im * row_tr);
-    const int zero_off = a.n_xd * tb;  // the row's zero trace
     const unsigned nxd = (unsigned)a.n_xd;
 
     if constexpr (!WINDOW && MODE == kForward) {
-      // Fast path.  Each row is read through a buffer resource whose base is
-      // the row; a gather's address is (scalar trace offset) + (lane sample
-      // offset).
-      const __amdgpu_buffer_rsrc_t rp =
-          __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(rowp), 0, (int)row_bytes, kRsrcWord3);
-      const __amdgpu_buffer_rsrc_t rm =
+      // Fast path.  Each present row is read through a buffer resource whose
+      // base is the row; a gather's address is (scalar trace offset) + (lane
+      // sample offset).  The crossline offsets dk in [0, kmax] are split into
+      // at most three ranges by which of the columns kj + dk, kj - dk hold
+      // data, and each range loads only the traces that exist: NR rows x NC
+      // columns per step.  Absent traces would add exact zeros, so the sum is
+      // the same as over all four partners.
+      const char* row0 = hp ? rowp : rowm;
+      const const_f32_ptr sc0 = hp ? scp : scm;
+      const bool two_rows = hp && hm;
+      const __amdgpu_buffer_rsrc_t rs0 =
+          __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(row0), 0, (int)row_bytes, kRsrcWord3);
+      const __amdgpu_buffer_rsrc_t rs1 =
           __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(rowm), 0, (int)row_bytes, kRsrcWord3);
-      auto prep = [&](int dk, Step<ST>& g) {
-        const float dyv = (float)dk * a.dy;
-        const float r2 = __builtin_fmaf(dyv, dyv, d2);
-        const int xp_ = kj + dk - xd_lo, xm_ = kj - dk - xd_lo;  // data crossline indices
-        const bool vp = (unsigned)xp_ < nxd;
-        const bool vm = dk > 0 && (unsigned)xm_ < nxd;
-        const int op = vp ? xp_ * tb : zero_off;
-        const int om = vm ? xm_ * tb : zero_off;
-        const float tau = sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq));
-        const float q = tau * a.inv_dt;
-        const int voff = min((int)q, nt1) * S::kBytes;
-        g.x0 = S::load(rp, voff, op);
-        g.x1 = S::load(rp, voff, om);
-        g.x2 = S::load(rm, voff, op);
-        g.x3 = S::load(rm, voff, om);
-        if constexpr (S::kScaled) {
-          const int sp = vp ? xp_ : a.n_xd, sm = vm ? xm_ : a.n_xd;
-          g.s0 = scp[sp];
-          g.s1 = scp[sm];
-          g.s2 = scm[sp];
-          g.s3 = scm[sm];
+
+      // Pipelined sweep of dk in [lo, hi] with NR rows and NC columns: the
+      // columns are kj + sgn*dk (NC = 1) or kj + dk, kj - dk (NC = 2).  The
+      // gathers of step dk+1 are issued before step dk is consumed; the
+      // scheduling barriers keep the compiler from hoisting the consumption
+      // above them.  A step prefetched past `hi` is never consumed; its
+      // loads read the zero trace or lie inside the row.
+      auto sweep = [&](auto nr_c, auto nc_c, int lo, int hi, int sgn) {
+        constexpr int NR = decltype(nr_c)::value;
+        constexpr int NC = decltype(nc_c)::value;
+        struct G {
+          float f, w;
+          bool m;
+          typename S::Raw x[NR * NC];
+          float sc[NR * NC];
+        };
+        auto prep = [&](int dk, G& g) {
+          const float dyv = (float)dk * a.dy;
+          const float r2 = __builtin_fmaf(dyv, dyv, d2);
+          int col[NC], off[NC];
+          col[0] = kj + (NC == 2 ? dk : sgn * dk) - xd_lo;
+          if constexpr (NC == 2) col[1] = kj - dk - xd_lo;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const bool ok = (unsigned)col[c] < nxd;
+            col[c] = ok ? col[c] : a.n_xd;
+            off[c] = col[c] * tb;
+          }
+          const float tau = sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq));
+          const float q = tau * a.inv_dt;
+          const int voff = min((int)q, nt1) * S::kBytes;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) g.x[c] = S::load(rs0, voff, off[c]);
+          if constexpr (NR == 2) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) g.x[NC + c] = S::load(rs1, voff, off[c]);
+          }
+          if constexpr (S::kScaled) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) g.sc[c] = sc0[col[c]];
+            if constexpr (NR == 2) {
+#pragma unroll
+              for (int c = 0; c < NC; ++c) g.sc[NC + c] = scm[col[c]];
+            }
+          }
+          g.f = __builtin_amdgcn_fractf(q);
+          g.w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+          g.m = r2 <= p.R2;
+        };
+        // a masked step adds 0 * w: the gathered sum is selected away before
+        // the multiply, so a non-finite sample outside the aperture cannot
+        // reach acc.  Slot order (row, column) is that of the four-partner
+        // sum (+db,+dk), (+db,-dk), (-db,+dk), (-db,-dk) with absent slots
+        // left out.
+        auto consume = [&](const G& g) {
+          float l[NR * NC];
+#pragma unroll
+          for (int k = 0; k < NR * NC; ++k) l[k] = S::lerp(g.x[k], g.f);
+          float sum;
+          if constexpr (S::kScaled) {
+            sum = l[0] * g.sc[0];
+#pragma unroll
+            for (int k = 1; k < NR * NC; ++k) sum = __builtin_fmaf(l[k], g.sc[k], sum);
+          } else if constexpr (NR * NC == 4) {
+            sum = (l[0] + l[1]) + (l[2] + l[3]);
+          } else if constexpr (NR * NC == 2) {
+            sum = l[0] + l[1];
+          } else {
+            sum = l[0];
+          }
+          sum = g.m ? sum : 0.0f;
+          acc = __builtin_fmaf(sum, g.w, acc);
+        };
+        if (lo > hi) return;
+        G A, B;
+        int dk = lo;
+        prep(dk, A);
+        for (; dk + 1 <= hi; dk += 2) {
+          prep(dk + 1, B);
+          __builtin_amdgcn_sched_barrier(0);
+          consume(A);
+          prep(dk + 2, A);
+          __builtin_amdgcn_sched_barrier(0);
+          consume(B);
         }
-        g.f = __builtin_amdgcn_fractf(q);
-        g.w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
-        g.m = r2 <= p.R2;
+        if (dk <= hi) consume(A);
       };
-      // a masked step adds 0 * w: the gathered sum is selected away before
-      // the multiply, so a non-finite sample outside the aperture cannot
-      // reach acc
-      auto consume = [&](const Step<ST>& g) {
-        const float l0 = S::lerp(g.x0, g.f);
-        const float l1 = S::lerp(g.x1, g.f);
-        const float l2 = S::lerp(g.x2, g.f);
-        const float l3 = S::lerp(g.x3, g.f);
-        float sum;
-        if constexpr (S::kScaled)
-          sum = __builtin_fmaf(l3, g.s3, __builtin_fmaf(l2, g.s2, __builtin_fmaf(l1, g.s1, l0 * g.s0)));
+      auto sweep_rows = [&](auto nc_c, int lo, int hi, int sgn) {
+        if (two_rows)
+          sweep(std::integral_constant<int, 2>{}, nc_c, lo, hi, sgn);
         else
-          sum = (l0 + l1) + (l2 + l3);
-        sum = g.m ? sum : 0.0f;
-        acc = __builtin_fmaf(sum, g.w, acc);
+          sweep(std::integral_constant<int, 1>{}, nc_c, lo, hi, sgn);
       };
-      // Two steps in flight: the gathers of step dk+1 are issued before step
-      // dk is consumed (the scheduling barriers keep the compiler from
-      // hoisting the consumption above them).  Steps past kmax are prefetched
-      // and consumed unconditionally, so the load counter is the same on
-      // every path; such a step either has r^2 > R2max (every lane masked or
-      // past the end of its trace: a zero term) or lies beyond the data on
-      // both sides (reads the zero trace).
-      Step<ST> A, B;
-      prep(0, A);
-      for (int dk = 0; dk <= kmax; dk += 2) {
-        prep(dk + 1, B);
-        __builtin_amdgcn_sched_barrier(0);
-        consume(A);
-        prep(dk + 2, A);
-        __builtin_amdgcn_sched_barrier(0);
-        consume(B);
+      // dk ranges with data: +dk side [p_lo, p_hi], -dk side [m_lo, m_hi]
+      const int p_lo = max(0, xd_lo - kj), p_hi = min(kmax, xd_hi - 1 - kj);
+      const int m_lo = max(1, kj - (xd_hi - 1)), m_hi = min(kmax, kj - xd_lo);
+      const int b_lo = max(p_lo, m_lo), b_hi = min(p_hi, m_hi);
+      const std::integral_constant<int, 1> one{};
+      const std::integral_constant<int, 2> two{};
+      if (b_lo > b_hi) {  // no offset with both columns: the sides in dk order
+        if (p_lo <= m_lo) {
+          sweep_rows(one, p_lo, p_hi, 1);
+          sweep_rows(one, m_lo, m_hi, -1);
+        } else {
+          sweep_rows(one, m_lo, m_hi, -1);
+          sweep_rows(one, p_lo, p_hi, 1);
+        }
+      } else {
+        if (p_lo < m_lo) sweep_rows(one, p_lo, b_lo - 1, 1);
+        else sweep_rows(one, m_lo, b_lo - 1, -1);
+        sweep_rows(two, b_lo, b_hi, 0);
+        if (p_hi > m_hi) sweep_rows(one, b_hi + 1, p_hi, 1);
+        else sweep_rows(one, b_hi + 1, m_hi, -1);
       }
     } else {
       for (int dk = 0; dk <= kmax; ++dk) {
