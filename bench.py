@@ -48,7 +48,7 @@ def parse():
                     help="skip the FP16/BF16 storage variants of the production line")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-rows", type=int, default=16, help="output inlines in the CPU sample")
-    ap.add_argument("--cpu-xl", type=int, default=96, help="output crosslines per sampled inline")
+    ap.add_argument("--cpu-xl", type=int, default=192, help="output crosslines per sampled inline")
     ap.add_argument("--copy-calibration", action="store_true")
     return ap.parse_args()
 
@@ -306,7 +306,10 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "migrate_kernel<false,false>", "kernel_ms": round(avg_kernel_s * 1e3, 3),
+                "kernel": "migrate_kernel (forward, FP%s traces)" % ("32" if args.storage == "f32" else "16"),
+                "kernel_ms": round(avg_kernel_s * 1e3, 3),
+                "kernel_ms_is": "sum of the step's migrate_kernel launches (aperture-row passes), "
+                                "HIP events on the launch stream",
                 "algorithmic_bytes": alg_bytes,
                 "note": f"algorithmic bytes = {pair_bytes:.0f} B per gathering pair (P_g); the kernel reuses "
                         "each trace window from L1/L2 across neighbouring output traces, so "

@@ -216,8 +216,9 @@ int psm_count_pairs(psm_ctx* ctx, const psm_survey* survey, const float* vel,
  * waves_per_block = output crosslines per workgroup (1, 2, 4, 8 or 16);
  * rows_per_pass = aperture row offsets summed per launch (0 = the whole
  * aperture in one launch; smaller values bound each launch's cache working
- * set to a few aperture rows, later launches accumulate).  Results do not
- * depend on the shape beyond FP32 summation order. */
+ * set to a few aperture rows, later launches accumulate; -1, the default =
+ * automatic: 8 for FP32 traces, one launch for FP16/BF16 traces).  Results
+ * do not depend on the shape beyond FP32 summation order. */
 int psm_set_tuning(psm_ctx* ctx, int waves_per_block, int rows_per_pass);
 int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
 

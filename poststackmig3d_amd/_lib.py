@@ -172,7 +172,7 @@ class Context:
         self.check(self.lib.psm_set_velocity_scale(self.h, float(alpha), float(am)),
                    "psm_set_velocity_scale")
 
-    def set_tuning(self, waves_per_block: int = 4, rows_per_pass: int = 0):
+    def set_tuning(self, waves_per_block: int = 4, rows_per_pass: int = -1):
         self.check(self.lib.psm_set_tuning(self.h, int(waves_per_block), int(rows_per_pass)),
                    "psm_set_tuning")
 

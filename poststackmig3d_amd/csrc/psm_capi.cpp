@@ -627,7 +627,7 @@ int psm_set_tuning(psm_ctx* c, int waves_per_block, int rows_per_pass) {
   if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4 &&
       waves_per_block != 8 && waves_per_block != 16)
     return fail(c, PSM_ERR_ARG, "waves_per_block must be 1, 2, 4, 8 or 16");
-  if (rows_per_pass < 0) return fail(c, PSM_ERR_ARG, "rows_per_pass must be >= 0");
+  if (rows_per_pass < -1) return fail(c, PSM_ERR_ARG, "rows_per_pass must be >= -1");
   std::lock_guard<std::mutex> lk(c->mu);
   c->tune.waves_per_block = waves_per_block;
   c->tune.rows_per_pass = rows_per_pass;

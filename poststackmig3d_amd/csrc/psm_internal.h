@@ -63,9 +63,17 @@ struct BatchSeg {
 };
 
 // Launch-shape parameters (chosen by the autotuner or the caller).
+// Row offsets per launch chosen by rows_per_pass = -1 for FP32 traces: the
+// FP32 gathers are bound by the cache hierarchy, and passes of 8 aperture rows
+// keep the rows that concurrently running workgroups read within L2 (1.2x on
+// the 401 x 401 x 1001 survey); half-precision traces are VALU-bound and run
+// the whole aperture in one launch.
+constexpr int kAutoRowsF32 = 8;
+
 struct Tuning {
   int waves_per_block = 4;   // output crosslines per workgroup (1, 2, 4, 8, 16)
-  int rows_per_pass = 0;     // aperture row offsets per launch; 0 = all in one launch
+  int rows_per_pass = -1;    // aperture row offsets per launch; 0 = all in one launch,
+                             // -1 = automatic (kAutoRowsF32 for FP32 traces, else 0)
 };
 
 // Launchers (psm_kernels.hip).  All enqueue on `stream` and return the HIP

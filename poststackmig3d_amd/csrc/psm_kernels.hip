@@ -747,7 +747,9 @@ hipError_t launch_migrate(const MigArgs& a0, int mode, hipStream_t stream, const
   MigArgs a = a0;
   // aperture-row passes: pass p sums the row offsets [p*R, (p+1)*R) (generic
   // rows: row indices); passes after the first accumulate into `out`
-  const int R = (mode == kCount || tune.rows_per_pass <= 0) ? (row_extent + 1) : tune.rows_per_pass;
+  const bool f32_traces = mode != kForward || a0.storage == PSM_STORE_F32;
+  const int rpp = tune.rows_per_pass >= 0 ? tune.rows_per_pass : (f32_traces ? kAutoRowsF32 : 0);
+  const int R = (mode == kCount || rpp <= 0) ? (row_extent + 1) : rpp;
   const int wpb = tune.waves_per_block;
   for (int lo = 0, pass = 0; lo <= row_extent; lo += R, ++pass) {
     a.d_lo = lo;
