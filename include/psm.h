@@ -239,6 +239,32 @@ typedef struct psm_timing {
 int psm_enable_timing(psm_ctx* ctx, int on);
 int psm_last_timing(psm_ctx* ctx, psm_timing* timing);
 
+/* Table velocity provider on the GPU.  Replaces
+ * TableVelocityProvider::interpolateVelocity / getVelocity
+ * (velocity_reader.cpp:320-370) evaluated for every sample of the data grid:
+ * vel[il][xl][t] at (il_labels[il], crossline_labels[xl], t * dt) is the
+ * first point at 4-D distance < 1e-10 (distance over inline and crossline
+ * label differences and time, in double), else the mean of the points'
+ * velocities with float weights 1 / (1 + d) accumulated in point order, and
+ * the smallest point velocity where that mean is not positive.  `points` is
+ * host memory; `vel` is host or device memory per `where`. */
+typedef struct psm_velocity_point {
+    int32_t inline_label;
+    int32_t crossline_label;
+    double time;
+    float velocity;
+} psm_velocity_point;
+int psm_table_velocity(psm_ctx* ctx, const psm_velocity_point* points, size_t n_points,
+                       const int32_t* inline_labels, size_t n_il,
+                       const int32_t* crossline_labels, size_t n_xl,
+                       size_t n_t, double dt, float* vel, int where);
+
+/* Page-locked host memory for whole-volume ingest (no reference
+ * counterpart): a cube read once into such a buffer reaches the device in one
+ * DMA transfer at full link rate.  psm_free_host(NULL) is a no-op. */
+int psm_alloc_host(size_t bytes, void** ptr);
+void psm_free_host(void* ptr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,12 @@ class BatchItem(C.Structure):
                 ("out", C.c_void_p), ("alpha", C.c_float)]
 
 
+class VelocityPoint(C.Structure):
+    """psm_velocity_point (include/psm.h)."""
+    _fields_ = [("inline_label", C.c_int32), ("crossline_label", C.c_int32),
+                ("time", C.c_double), ("velocity", C.c_float)]
+
+
 class Timing(C.Structure):
     """psm_timing (include/psm.h)."""
     _fields_ = [("pack_ms", C.c_float), ("migrate_ms", C.c_float)]
@@ -95,6 +101,11 @@ SIGNATURES = [
                                      C.POINTER(C.c_float)]),
     ("psm_batch_run", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     ("psm_batch_destroy", None, [C.c_void_p]),
+    ("psm_table_velocity", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                                     C.c_void_p, C.c_size_t, C.c_size_t, C.c_double, C.c_void_p,
+                                     C.c_int]),
+    ("psm_alloc_host", C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("psm_free_host", None, [C.c_void_p]),
     ("psm_set_storage", C.c_int, [C.c_void_p, C.c_int]),
     ("psm_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("psm_get_tuning", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

@@ -52,7 +52,7 @@ struct psm_ctx {
   hipStream_t stream = nullptr;
   std::mutex mu;
   std::string err;
-  DevBuf raw, pairs, scale, vel, out, t0, tasks, rowpos, counters, dot, dotx, doty;
+  DevBuf raw, pairs, scale, vel, out, t0, tasks, rowpos, counters, dot, dotx, doty, tvpts, tvlab;
   int storage = PSM_STORE_F32;
   // kernel-level timing of the last volume call (psm_enable_timing)
   bool timing = false;
@@ -595,6 +595,53 @@ int psm_dot_f64(psm_ctx* c, const float* x, const float* y, size_t n, int where,
                             hipMemcpyDeviceToHost, st));
   PSM_HIP(c, hipStreamSynchronize(st));
   return PSM_OK;
+}
+
+int psm_table_velocity(psm_ctx* c, const psm_velocity_point* points, size_t n_points,
+                       const int32_t* il, size_t n_il, const int32_t* xl, size_t n_xl, size_t n_t,
+                       double dt, float* vel, int where) {
+  if (!c) return PSM_ERR_ARG;
+  if (!points || n_points == 0) return fail(c, PSM_ERR_ARG, "No valid data found in velocity table");
+  if (!il || !xl || !vel) return fail(c, PSM_ERR_ARG, "null labels or output");
+  if (where != PSM_HOST && where != PSM_DEVICE) return fail(c, PSM_ERR_ARG, "bad `where`");
+  if (n_points > (size_t)INT32_MAX || n_il > (1u << 20) || n_xl > (1u << 20) || n_t > (1u << 20))
+    return fail(c, PSM_ERR_ARG, "table velocity: size out of range");
+  std::lock_guard<std::mutex> lk(c->mu);
+  PSM_HIP(c, hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  float vmin = points[0].velocity;
+  for (size_t k = 1; k < n_points; ++k) vmin = std::min(vmin, points[k].velocity);
+  PSM_HIP(c, upload(c->tvpts, points, n_points * sizeof(psm_velocity_point), st));
+  std::vector<int32_t> lab(il, il + n_il);
+  lab.insert(lab.end(), xl, xl + n_xl);
+  PSM_HIP(c, upload(c->tvlab, lab.data(), lab.size() * sizeof(int32_t), st));
+  const size_t n = n_il * n_xl * n_t;
+  float* dvel = vel;
+  if (where == PSM_HOST) {
+    PSM_ALLOC(c, c->out, n * sizeof(float));
+    dvel = c->out.as<float>();
+  }
+  const int32_t* dl = c->tvlab.as<int32_t>();
+  PSM_HIP(c, psm::launch_table_velocity(c->tvpts.as<psm_velocity_point>(), (int)n_points, vmin, dl,
+                                        dl + n_il, (int)n_il, (int)n_xl, (int)n_t, dt, dvel, st));
+  if (where == PSM_HOST)
+    PSM_HIP(c, hipMemcpyAsync(vel, dvel, n * sizeof(float), hipMemcpyDeviceToHost, st));
+  PSM_HIP(c, hipStreamSynchronize(st));
+  return PSM_OK;
+}
+
+int psm_alloc_host(size_t bytes, void** ptr) {
+  if (!ptr) return PSM_ERR_ARG;
+  *ptr = nullptr;
+  if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    *ptr = nullptr;
+    return PSM_ERR_NOMEM;
+  }
+  return PSM_OK;
+}
+
+void psm_free_host(void* ptr) {
+  if (ptr) (void)hipHostFree(ptr);
 }
 
 int psm_set_velocity_scale(psm_ctx* c, float alpha, float alpha_mask) {

@@ -89,6 +89,12 @@ hipError_t launch_pack(int storage, const float* x, void* xp, float* scale, int 
 // tune.rows_per_pass row offsets.
 hipError_t launch_migrate(const MigArgs& a, int mode, hipStream_t stream, const Tuning& tune,
                           int row_extent);
+// Table velocity provider: vel[n_il][n_xl][n_t] from `n_pts` points
+// (device memory), labels (device memory), minimum point velocity `vmin`.
+hipError_t launch_table_velocity(const psm_velocity_point* pts, int n_pts, float vmin,
+                                 const int32_t* il, const int32_t* xl, int n_il, int n_xl,
+                                 int n_t, double dt, float* vel, hipStream_t stream);
+
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream);
 // Persistent batch launch (forward migration of several volumes): `blocks`
 // workgroups of wpb waves drain `total` queue positions described by segs.

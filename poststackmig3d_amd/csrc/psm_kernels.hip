@@ -683,6 +683,47 @@ __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, 
   }
 }
 
+// Table velocity provider (TableVelocityProvider::interpolateVelocity,
+// velocity_reader.cpp:320-361): one thread per output sample, points in
+// order, the 4-D distance in double, float weights and sums; the first point
+// within 1e-10 gives the value directly.  Every thread reads the same point
+// at a time, so the point loads are uniform.
+__global__ __launch_bounds__(256) void table_velocity_kernel(
+    const psm_velocity_point* __restrict__ pts, int n_pts, float vmin,
+    const int32_t* __restrict__ il, const int32_t* __restrict__ xl, int n_il, int n_xl, int n_t,
+    double dt, float* __restrict__ vel) {
+  const size_t n = (size_t)n_il * n_xl * n_t;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += stride) {
+    const int t = (int)(idx % n_t);
+    const size_t r = idx / n_t;
+    const int32_t xv = xl[r % n_xl], iv = il[r / n_xl];
+    const double time = (double)t * dt;
+    float sv = 0.0f, sw = 0.0f, out = 0.0f;
+    bool exact = false;
+    for (int k = 0; k < n_pts; ++k) {
+      const psm_velocity_point p = pts[k];
+      const double di = fabs((double)(p.inline_label - iv));
+      const double dx = fabs((double)(p.crossline_label - xv));
+      const double dtt = fabs(p.time - time);
+      const double d = sqrt(di * di + dx * dx + dtt * dtt);
+      if (d < 1e-10) {
+        out = p.velocity;
+        exact = true;
+        break;
+      }
+      const float w = 1.0f / (1.0f + (float)d);
+      sv = sv + p.velocity * w;
+      sw = sw + w;
+    }
+    if (!exact) {
+      out = sw > 0.0f ? sv / sw : vmin;
+      if (out <= 0.0f) out = vmin;
+    }
+    vel[idx] = out;
+  }
+}
+
 }  // namespace
 
 size_t pairs_bytes(int storage, int n_rows, int n_xd, int n_t) {
@@ -801,6 +842,18 @@ hipError_t launch_batch(int storage, const MigArgs* items, const BatchSeg* segs,
     case PSM_STORE_BF16: return launch_batch_st<PSM_STORE_BF16>(items, segs, n_segs, counter, total, wpb, blocks, stream);
     default: return launch_batch_st<PSM_STORE_F32>(items, segs, n_segs, counter, total, wpb, blocks, stream);
   }
+}
+
+hipError_t launch_table_velocity(const psm_velocity_point* pts, int n_pts, float vmin,
+                                 const int32_t* il, const int32_t* xl, int n_il, int n_xl,
+                                 int n_t, double dt, float* vel, hipStream_t stream) {
+  const size_t n = (size_t)n_il * n_xl * n_t;
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 262144) blocks = 262144;
+  hipLaunchKernelGGL(table_velocity_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, pts, n_pts,
+                     vmin, il, xl, n_il, n_xl, n_t, dt, vel);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream) {
