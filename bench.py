@@ -46,6 +46,10 @@ def parse():
                     default="production")
     ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
                     help="trace storage of the main line (compute stays FP32)")
+    ap.add_argument("--kernel", choices=["auto", "staged", "gather"], default="auto",
+                    help="forward kernel form: chosen by trace storage (auto), aperture "
+                         "windows staged in LDS (8 crosslines per workgroup, 2 offsets per "
+                         "group, one launch), or gathered through the vector memory path")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP16/BF16 storage variants of the production line")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -308,6 +312,11 @@ def main():
     stream = torch.cuda.Stream(device=dev)
     ctx = P.context(0)
     ctx.set_stream(stream.cuda_stream)
+    if args.kernel == "staged":
+        ctx.set_tuning(8, 0)
+        ctx.set_staging(True, 2)
+    elif args.kernel == "gather":
+        ctx.set_staging(False)
 
     if args.workload == "production":
         shape = (401, 401, 1001)
@@ -374,8 +383,10 @@ def main():
         try:
             with open(PMC_FILE) as f:
                 pmc = json.load(f)
-            if pmc.get("workload") == args.workload and args.storage in pmc:
-                ent = pmc[args.storage]
+            form = "staged" if (args.kernel == "staged" or
+                                (args.kernel == "auto" and args.storage == "f32")) else "gather"
+            ent = pmc.get(args.storage) if pmc.get("workload") == args.workload else None
+            if ent and ent.get("form", "gather") == form:
                 traffic = ent.get("traffic_bytes_per_step")
                 counters = {k: ent[k] for k in ("valu_busy", "ta_busy", "td_busy", "l2_hit")
                             if k in ent}
@@ -383,7 +394,10 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "migrate_kernel (forward, FP%s traces)" % ("32" if args.storage == "f32" else "16"),
+                "kernel": "%s (forward, FP%s traces)" % (
+                    "migrate_lds_kernel" if (args.kernel == "staged" or
+                                             (args.kernel == "auto" and args.storage == "f32"))
+                    else "migrate_kernel", "32" if args.storage == "f32" else "16"),
                 "kernel_ms": round(avg_kernel_s * 1e3, 3),
                 "kernel_ms_is": "sum of the step's migrate_kernel launches (aperture-row passes), "
                                 "HIP events on the launch stream",
@@ -436,7 +450,7 @@ def main():
         "data": "synthetic (seeded, generated on the GPU)",
         "config": {"workload": desc, "n_il": shape[0], "n_xl": shape[1], "n_t": shape[2],
                    "out": [info.n_out_il, info.n_out_xl, info.n_t], "p_ap": p_ap, "p_g": p_g,
-                   "trace_storage": args.storage,
+                   "trace_storage": args.storage, "kernel_form": args.kernel,
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -222,6 +222,19 @@ int psm_count_pairs(psm_ctx* ctx, const psm_survey* survey, const float* vel,
 int psm_set_tuning(psm_ctx* ctx, int waves_per_block, int rows_per_pass);
 int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
 
+/* Forward-migration kernel form (no reference counterpart): on = 1 stages
+ * each aperture trace's time window in LDS once per workgroup of output
+ * crosslines (waves_per_block 4 or 8) and gathers from there; on = 0 gathers
+ * every sample through the vector memory path; on = -1 (the default) stages
+ * FP32 traces with 8 output crosslines per workgroup and 2 crossline offsets
+ * per staged group in one launch, and gathers FP16/BF16 traces.  steps (2 or
+ * 4) = crossline offsets per staged group when on = 1.  The staged form
+ * applies to whole-volume calls and per-inline calls with contiguous rows;
+ * other calls use the gather form.  Results do not depend on the form beyond
+ * FP32 summation order. */
+int psm_set_staging(psm_ctx* ctx, int on, int steps);
+int psm_get_staging(psm_ctx* ctx, int* on, int* steps);
+
 /* Trace storage used by later calls on this context (no reference
  * counterpart): PSM_STORE_F32 (default), PSM_STORE_F16 or PSM_STORE_BF16.
  * The reduced forms halve the bytes each gather reads; results then differ

@@ -109,6 +109,8 @@ SIGNATURES = [
     ("psm_set_storage", C.c_int, [C.c_void_p, C.c_int]),
     ("psm_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("psm_get_tuning", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("psm_set_staging", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("psm_get_staging", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("psm_enable_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("psm_last_timing", C.c_int, [C.c_void_p, C.POINTER(Timing)]),
 ]
@@ -192,6 +194,18 @@ class Context:
         self.check(self.lib.psm_get_tuning(self.h, C.byref(w), C.byref(r)), "psm_get_tuning")
         return w.value, r.value
 
+
+    def set_staging(self, on=None, steps: int = 2):
+        """Forward kernel form: aperture windows staged in LDS (True), gathered
+        through the vector memory path (False), or chosen by trace storage
+        (None, the default); steps = crossline offsets per staged group (2 or 4)."""
+        mode = -1 if on is None else (1 if on else 0)
+        self.check(self.lib.psm_set_staging(self.h, mode, int(steps)), "psm_set_staging")
+
+    def get_staging(self):
+        on, st = C.c_int(), C.c_int()
+        self.check(self.lib.psm_get_staging(self.h, C.byref(on), C.byref(st)), "psm_get_staging")
+        return (None if on.value < 0 else bool(on.value)), st.value
     def enable_timing(self, on: bool = True):
         self.check(self.lib.psm_enable_timing(self.h, 1 if on else 0), "psm_enable_timing")
 

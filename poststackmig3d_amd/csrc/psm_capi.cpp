@@ -689,6 +689,24 @@ int psm_get_tuning(psm_ctx* c, int* waves_per_block, int* rows_per_pass) {
   return PSM_OK;
 }
 
+int psm_set_staging(psm_ctx* c, int on, int steps) {
+  if (!c) return PSM_ERR_ARG;
+  if (on < -1 || on > 1) return fail(c, PSM_ERR_ARG, "staging must be -1, 0 or 1");
+  if (steps != 2 && steps != 4) return fail(c, PSM_ERR_ARG, "steps must be 2 or 4");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->tune.staging = on;
+  c->tune.stage_steps = steps;
+  return PSM_OK;
+}
+
+int psm_get_staging(psm_ctx* c, int* on, int* steps) {
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (on) *on = c->tune.staging;
+  if (steps) *steps = c->tune.stage_steps;
+  return PSM_OK;
+}
+
 int psm_enable_timing(psm_ctx* c, int on) {
   if (!c) return PSM_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);

@@ -74,7 +74,19 @@ struct Tuning {
   int waves_per_block = 4;   // output crosslines per workgroup (1, 2, 4, 8, 16)
   int rows_per_pass = -1;    // aperture row offsets per launch; 0 = all in one launch,
                              // -1 = automatic (kAutoRowsF32 for FP32 traces, else 0)
+  int staging = -1;          // forward migration: 1 = aperture windows staged in LDS
+                             // (workgroups of 4 or 8 waves, contiguous rows), 0 = gathers,
+                             // -1 = automatic (kAutoStaging for FP32 traces, else gathers)
+  int stage_steps = 2;       // crossline offsets per staged group (2 or 4)
 };
+
+// Automatic forward-kernel choice for FP32 traces: windows staged in LDS by
+// workgroups of 8 output crosslines, 2 crossline offsets per staged group, the
+// whole aperture in one launch (on the 401 x 401 x 1001 survey this beats the
+// gather form, which is bound by the texture-address rate); half-precision
+// traces are VALU-bound in either form and use the gather form.
+constexpr int kAutoStageWaves = 8;
+constexpr int kAutoStageSteps = 2;
 
 // Launchers (psm_kernels.hip).  All enqueue on `stream` and return the HIP
 // status of the launch.

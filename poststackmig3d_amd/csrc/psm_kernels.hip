@@ -69,9 +69,33 @@ __device__ __forceinline__ float sqrt_rn(float x) {
   return r;
 }
 
+// Square root that is correctly rounded except in rare cases: v_sqrt_f32 on
+// gfx950 returns the correctly rounded root or the float just below it
+// (measured over 6.7e7 arguments in [1e-6, 1e2]: 15.4 % one below, 0.006 %
+// one above, none further off), so testing the upper neighbour alone gives
+// the correctly rounded root for all but ~6e-5 of the arguments, which come
+// out one ulp high.
+__device__ __forceinline__ float sqrt_up(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float s_up = __int_as_float(__float_as_int(s) + 1);
+  return (__builtin_fmaf(-s_up, s, x) > 0.0f) ? s_up : s;
+}
+
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
   return v;
 }
 
@@ -147,22 +171,25 @@ struct LaneParams {
   float mval;  // adjoint: the image sample this lane scatters
 };
 
-// Sums the aperture of one (output trace, t0 tile) over the row offsets
-// |db| in [d_lo, d_hi) of this pass.  WINDOW = some lane uses parameters of
-// an earlier crossline whose crossline window can cut the aperture (sticky
-// samples of migration_kernel.cpp:64-152), so each gather is also checked
-// against that window.  MODE: kForward sums, kCount counts pairs, kAdjoint
-// scatters the lane's image sample into the data cube with the same
-// (i, f, w, mask) as the forward sum, kTangent sums the derivative of each
-// term with respect to the velocity scale.  All loop
-// bounds and trace offsets are wave-uniform (scalar registers); absent
-// crosslines read the row's zero trace and absent rows the zero row, so the
-// four gathers of a step are issued back to back, and the fast path loads
-// step dk+1 before it consumes step dk (two steps of gathers in flight).
-template <int ST, bool GENERIC_ROWS, bool WINDOW, int MODE>
-__device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, int kj,
-                                             const LaneParams& p, float R2max,
-                                             float& acc, unsigned& cap, unsigned& cg) {
+// Sums one aperture row pair of one (output trace, t0 tile): rows +db and
+// -db of the output inline (rowp / rowm, present when hp / hm; ip / im their
+// index in the scale table), d2 = (db dx)^2, over the crossline offsets with
+// r^2 <= R2max.  WINDOW = some lane uses parameters of an earlier crossline
+// whose crossline window can cut the aperture (sticky samples of
+// migration_kernel.cpp:64-152), so each gather is also checked against that
+// window.  MODE: kForward sums, kCount counts pairs, kAdjoint scatters the
+// lane's image sample into the data cube with the same (i, f, w, mask) as the
+// forward sum, kTangent sums the derivative of each term with respect to the
+// velocity scale.  All loop bounds and trace offsets are wave-uniform (scalar
+// registers); absent crosslines read the row's zero trace and absent rows the
+// zero row, so the four gathers of a step are issued back to back, and the
+// fast path loads step dk+1 before it consumes step dk (two steps of gathers
+// in flight).
+template <int ST, bool WINDOW, int MODE>
+__device__ __forceinline__ void sweep_row(const MigArgs& a, int kj, const LaneParams& p,
+                                          float R2max, float& acc, unsigned& cap, unsigned& cg,
+                                          const char* rowp, bool hp, int ip, const char* rowm,
+                                          bool hm, int im, float d2) {
   using S = Store<ST>;
   constexpr bool COUNT = MODE == kCount;
   const int n_t = a.n_t;
@@ -171,247 +198,259 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
   // pair layout rows are [n_xd + 1][n_t]: the last trace of every row is zero
   const int tb = n_t * S::kBytes;
   const size_t row_bytes = (size_t)(a.n_xd + 1) * tb;
-  const char* const base = reinterpret_cast<const char*>(a.xp);
-  const char* const zero_row = base + (size_t)a.n_rows * row_bytes;
   const int row_tr = a.n_xd + 1;  // traces per row (scales)
   // largest crossline offset that can hold data
   const int dk_lim = max(kj - xd_lo, xd_hi - 1 - kj);
 
-  // rowp / rowm: rows at +db and -db (present when hp / hm), with their
-  // index for the scale table
-  auto row_sweep = [&](const char* rowp, bool hp, int ip, const char* rowm, bool hm, int im,
-                       float d2) {
-    // last dk with r^2 <= R2max (r^2 is nondecreasing in dk)
-    int kmax = (int)(__builtin_sqrtf(R2max - d2) * a.inv_dy) + 2;
-    kmax = min(kmax, dk_lim);
-    while (kmax >= 0) {
-      const float dyv = (float)kmax * a.dy;
-      if (__builtin_fmaf(dyv, dyv, d2) <= R2max) break;
-      --kmax;
-    }
-    kmax = uniform_i(kmax);
-    if (kmax < 0) return;
-    const const_f32_ptr scp = as_const(a.scale + (size_t)ip * row_tr);
-    const const_f32_ptr scm = as_const(a.scale + (size_t)im * row_tr);
-    const unsigned nxd = (unsigned)a.n_xd;
+  // last dk with r^2 <= R2max (r^2 is nondecreasing in dk)
+  int kmax = (int)(__builtin_sqrtf(R2max - d2) * a.inv_dy) + 2;
+  kmax = min(kmax, dk_lim);
+  while (kmax >= 0) {
+    const float dyv = (float)kmax * a.dy;
+    if (__builtin_fmaf(dyv, dyv, d2) <= R2max) break;
+    --kmax;
+  }
+  kmax = uniform_i(kmax);
+  if (kmax < 0) return;
+  const const_f32_ptr scp = as_const(a.scale + (size_t)ip * row_tr);
+  const const_f32_ptr scm = as_const(a.scale + (size_t)im * row_tr);
+  const unsigned nxd = (unsigned)a.n_xd;
 
-    if constexpr (!WINDOW && MODE == kForward) {
-      // Fast path.  Each present row is read through a buffer resource whose
-      // base is the row; a gather's address is (scalar trace offset) + (lane
-      // sample offset).  The crossline offsets dk in [0, kmax] are split into
-      // at most three ranges by which of the columns kj + dk, kj - dk hold
-      // data, and each range loads only the traces that exist: NR rows x NC
-      // columns per step.  Absent traces would add exact zeros, so the sum is
-      // the same as over all four partners.
-      const char* row0 = hp ? rowp : rowm;
-      const const_f32_ptr sc0 = hp ? scp : scm;
-      const bool two_rows = hp && hm;
-      const __amdgpu_buffer_rsrc_t rs0 =
-          __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(row0), 0, (int)row_bytes, kRsrcWord3);
-      const __amdgpu_buffer_rsrc_t rs1 =
-          __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(rowm), 0, (int)row_bytes, kRsrcWord3);
+  if constexpr (!WINDOW && MODE == kForward) {
+    // Fast path.  Each present row is read through a buffer resource whose
+    // base is the row; a gather's address is (scalar trace offset) + (lane
+    // sample offset).  The crossline offsets dk in [0, kmax] are split into
+    // at most three ranges by which of the columns kj + dk, kj - dk hold
+    // data, and each range loads only the traces that exist: NR rows x NC
+    // columns per step.  Absent traces would add exact zeros, so the sum is
+    // the same as over all four partners.
+    const char* row0 = hp ? rowp : rowm;
+    const const_f32_ptr sc0 = hp ? scp : scm;
+    const bool two_rows = hp && hm;
+    const __amdgpu_buffer_rsrc_t rs0 =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(row0), 0, (int)row_bytes, kRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rs1 =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(rowm), 0, (int)row_bytes, kRsrcWord3);
 
-      // Pipelined sweep of dk in [lo, hi] with NR rows and NC columns: the
-      // columns are kj + sgn*dk (NC = 1) or kj + dk, kj - dk (NC = 2).  The
-      // gathers of step dk+1 are issued before step dk is consumed; the
-      // scheduling barriers keep the compiler from hoisting the consumption
-      // above them.  A step prefetched past `hi` is never consumed; its
-      // loads read the zero trace or lie inside the row.
-      auto sweep = [&](auto nr_c, auto nc_c, int lo, int hi, int sgn) {
-        constexpr int NR = decltype(nr_c)::value;
-        constexpr int NC = decltype(nc_c)::value;
-        struct G {
-          float f, w;
-          bool m;
-          typename S::Raw x[NR * NC];
-          float sc[NR * NC];
-        };
-        auto prep = [&](int dk, G& g) {
-          const float dyv = (float)dk * a.dy;
-          const float r2 = __builtin_fmaf(dyv, dyv, d2);
-          int col[NC], off[NC];
-          col[0] = kj + (NC == 2 ? dk : sgn * dk) - xd_lo;
-          if constexpr (NC == 2) col[1] = kj - dk - xd_lo;
-#pragma unroll
-          for (int c = 0; c < NC; ++c) {
-            const bool ok = (unsigned)col[c] < nxd;
-            col[c] = ok ? col[c] : a.n_xd;
-            off[c] = col[c] * tb;
-          }
-          const float tau = sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq));
-          const float q = tau * a.inv_dt;
-          const int voff = min((int)q, nt1) * S::kBytes;
-#pragma unroll
-          for (int c = 0; c < NC; ++c) g.x[c] = S::load(rs0, voff, off[c]);
-          if constexpr (NR == 2) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) g.x[NC + c] = S::load(rs1, voff, off[c]);
-          }
-          if constexpr (S::kScaled) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) g.sc[c] = sc0[col[c]];
-            if constexpr (NR == 2) {
-#pragma unroll
-              for (int c = 0; c < NC; ++c) g.sc[NC + c] = scm[col[c]];
-            }
-          }
-          g.f = __builtin_amdgcn_fractf(q);
-          g.w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
-          g.m = r2 <= p.R2;
-        };
-        // a masked step adds 0 * w: the gathered sum is selected away before
-        // the multiply, so a non-finite sample outside the aperture cannot
-        // reach acc.  Slot order (row, column) is that of the four-partner
-        // sum (+db,+dk), (+db,-dk), (-db,+dk), (-db,-dk) with absent slots
-        // left out.
-        auto consume = [&](const G& g) {
-          float l[NR * NC];
-#pragma unroll
-          for (int k = 0; k < NR * NC; ++k) l[k] = S::lerp(g.x[k], g.f);
-          float sum;
-          if constexpr (S::kScaled) {
-            sum = l[0] * g.sc[0];
-#pragma unroll
-            for (int k = 1; k < NR * NC; ++k) sum = __builtin_fmaf(l[k], g.sc[k], sum);
-          } else if constexpr (NR * NC == 4) {
-            sum = (l[0] + l[1]) + (l[2] + l[3]);
-          } else if constexpr (NR * NC == 2) {
-            sum = l[0] + l[1];
-          } else {
-            sum = l[0];
-          }
-          sum = g.m ? sum : 0.0f;
-          acc = __builtin_fmaf(sum, g.w, acc);
-        };
-        if (lo > hi) return;
-        G A, B;
-        int dk = lo;
-        prep(dk, A);
-        for (; dk + 1 <= hi; dk += 2) {
-          prep(dk + 1, B);
-          __builtin_amdgcn_sched_barrier(0);
-          consume(A);
-          prep(dk + 2, A);
-          __builtin_amdgcn_sched_barrier(0);
-          consume(B);
-        }
-        if (dk <= hi) consume(A);
+    // Pipelined sweep of dk in [lo, hi] with NR rows and NC columns: the
+    // columns are kj + sgn*dk (NC = 1) or kj + dk, kj - dk (NC = 2).  The
+    // gathers of step dk+1 are issued before step dk is consumed; the
+    // scheduling barriers keep the compiler from hoisting the consumption
+    // above them.  A step prefetched past `hi` is never consumed; its
+    // loads read the zero trace or lie inside the row.
+    auto sweep = [&](auto nr_c, auto nc_c, int lo, int hi, int sgn) {
+      constexpr int NR = decltype(nr_c)::value;
+      constexpr int NC = decltype(nc_c)::value;
+      struct G {
+        float f, w;
+        bool m;
+        typename S::Raw x[NR * NC];
+        float sc[NR * NC];
       };
-      auto sweep_rows = [&](auto nc_c, int lo, int hi, int sgn) {
-        if (two_rows)
-          sweep(std::integral_constant<int, 2>{}, nc_c, lo, hi, sgn);
-        else
-          sweep(std::integral_constant<int, 1>{}, nc_c, lo, hi, sgn);
-      };
-      // dk ranges with data: +dk side [p_lo, p_hi], -dk side [m_lo, m_hi]
-      const int p_lo = max(0, xd_lo - kj), p_hi = min(kmax, xd_hi - 1 - kj);
-      const int m_lo = max(1, kj - (xd_hi - 1)), m_hi = min(kmax, kj - xd_lo);
-      const int b_lo = max(p_lo, m_lo), b_hi = min(p_hi, m_hi);
-      const std::integral_constant<int, 1> one{};
-      const std::integral_constant<int, 2> two{};
-      if (b_lo > b_hi) {  // no offset with both columns: the sides in dk order
-        if (p_lo <= m_lo) {
-          sweep_rows(one, p_lo, p_hi, 1);
-          sweep_rows(one, m_lo, m_hi, -1);
-        } else {
-          sweep_rows(one, m_lo, m_hi, -1);
-          sweep_rows(one, p_lo, p_hi, 1);
-        }
-      } else {
-        if (p_lo < m_lo) sweep_rows(one, p_lo, b_lo - 1, 1);
-        else sweep_rows(one, m_lo, b_lo - 1, -1);
-        sweep_rows(two, b_lo, b_hi, 0);
-        if (p_hi > m_hi) sweep_rows(one, b_hi + 1, p_hi, 1);
-        else sweep_rows(one, b_hi + 1, m_hi, -1);
-      }
-    } else {
-      for (int dk = 0; dk <= kmax; ++dk) {
+      auto prep = [&](int dk, G& g) {
         const float dyv = (float)dk * a.dy;
         const float r2 = __builtin_fmaf(dyv, dyv, d2);
-        const int kp = kj + dk, km = kj - dk;
-        const bool has_kp = kp >= xd_lo && kp < xd_hi;
-        const bool has_km = dk > 0 && km >= xd_lo && km < xd_hi;
-        if (!has_kp && !has_km) continue;
-        const bool in_mask = r2 <= p.R2;
+        int col[NC], off[NC];
+        col[0] = kj + (NC == 2 ? dk : sgn * dk) - xd_lo;
+        if constexpr (NC == 2) col[1] = kj - dk - xd_lo;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const bool ok = (unsigned)col[c] < nxd;
+          col[c] = ok ? col[c] : a.n_xd;
+          off[c] = col[c] * tb;
+        }
         const float tau = sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq));
         const float q = tau * a.inv_dt;
-        const int i = (int)q;
-        bool okp = true, okm = true;
-        if (WINDOW) {
-          okp = kp >= p.wlo && kp < p.whi;
-          okm = km >= p.wlo && km < p.whi;
+        const int voff = min((int)q, nt1) * S::kBytes;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) g.x[c] = S::load(rs0, voff, off[c]);
+        if constexpr (NR == 2) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) g.x[NC + c] = S::load(rs1, voff, off[c]);
         }
-        if constexpr (COUNT) {
-          const unsigned nrow = (hp ? 1u : 0u) + (hm ? 1u : 0u);
-          const unsigned n = nrow * ((has_kp && okp ? 1u : 0u) + (has_km && okm ? 1u : 0u));
-          cap += in_mask ? n : 0u;
-          cg += (in_mask && i < nt1) ? n : 0u;
-        } else if constexpr (MODE == kAdjoint) {
-          // d[b][k][i] += w (1 - f) m,  d[b][k][i+1] += w f m  (transpose of
-          // the interpolation fma(f, x[i+1] - x[i], x[i]))
-          if (in_mask && i < nt1) {
-            const float f = __builtin_amdgcn_fractf(q);
-            const float w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
-            const float wm = w * p.mval;
-            const float c0 = wm - wm * f, c1 = wm * f;
-            auto scatter = [&](bool row_ok, int row, bool k_ok, int k) {
-              if (row_ok && k_ok) {
-                float* d = a.adj_out + ((size_t)row * a.n_xd + (k - xd_lo)) * n_t + i;
-                unsafeAtomicAdd(d, c0);
-                unsafeAtomicAdd(d + 1, c1);
-              }
-            };
-            scatter(hp, ip, has_kp && okp, kp);
-            scatter(hp, ip, has_km && okm, km);
-            scatter(hm, im, has_kp && okp, kp);
-            scatter(hm, im, has_km && okm, km);
+        if constexpr (S::kScaled) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) g.sc[c] = sc0[col[c]];
+          if constexpr (NR == 2) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) g.sc[NC + c] = scm[col[c]];
           }
+        }
+        g.f = __builtin_amdgcn_fractf(q);
+        g.w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+        g.m = r2 <= p.R2;
+      };
+      // a masked step adds 0 * w: the gathered sum is selected away before
+      // the multiply, so a non-finite sample outside the aperture cannot
+      // reach acc.  Slot order (row, column) is that of the four-partner
+      // sum (+db,+dk), (+db,-dk), (-db,+dk), (-db,-dk) with absent slots
+      // left out.
+      auto consume = [&](const G& g) {
+        float l[NR * NC];
+#pragma unroll
+        for (int k = 0; k < NR * NC; ++k) l[k] = S::lerp(g.x[k], g.f);
+        float sum;
+        if constexpr (S::kScaled) {
+          sum = l[0] * g.sc[0];
+#pragma unroll
+          for (int k = 1; k < NR * NC; ++k) sum = __builtin_fmaf(l[k], g.sc[k], sum);
+        } else if constexpr (NR * NC == 4) {
+          sum = (l[0] + l[1]) + (l[2] + l[3]);
+        } else if constexpr (NR * NC == 2) {
+          sum = l[0] + l[1];
         } else {
-          const int xp_ = has_kp ? kp - xd_lo : a.n_xd;
-          const int xm_ = has_km ? km - xd_lo : a.n_xd;
-          const size_t off = (size_t)min(i, nt1) * S::kBytes;
+          sum = l[0];
+        }
+        sum = g.m ? sum : 0.0f;
+        acc = __builtin_fmaf(sum, g.w, acc);
+      };
+      if (lo > hi) return;
+      G A, B;
+      int dk = lo;
+      prep(dk, A);
+      for (; dk + 1 <= hi; dk += 2) {
+        prep(dk + 1, B);
+        __builtin_amdgcn_sched_barrier(0);
+        consume(A);
+        prep(dk + 2, A);
+        __builtin_amdgcn_sched_barrier(0);
+        consume(B);
+      }
+      if (dk <= hi) consume(A);
+    };
+    auto sweep_rows = [&](auto nc_c, int lo, int hi, int sgn) {
+      if (two_rows)
+        sweep(std::integral_constant<int, 2>{}, nc_c, lo, hi, sgn);
+      else
+        sweep(std::integral_constant<int, 1>{}, nc_c, lo, hi, sgn);
+    };
+    // dk ranges with data: +dk side [p_lo, p_hi], -dk side [m_lo, m_hi]
+    const int p_lo = max(0, xd_lo - kj), p_hi = min(kmax, xd_hi - 1 - kj);
+    const int m_lo = max(1, kj - (xd_hi - 1)), m_hi = min(kmax, kj - xd_lo);
+    const int b_lo = max(p_lo, m_lo), b_hi = min(p_hi, m_hi);
+    const std::integral_constant<int, 1> one{};
+    const std::integral_constant<int, 2> two{};
+    if (b_lo > b_hi) {  // no offset with both columns: the sides in dk order
+      if (p_lo <= m_lo) {
+        sweep_rows(one, p_lo, p_hi, 1);
+        sweep_rows(one, m_lo, m_hi, -1);
+      } else {
+        sweep_rows(one, m_lo, m_hi, -1);
+        sweep_rows(one, p_lo, p_hi, 1);
+      }
+    } else {
+      if (p_lo < m_lo) sweep_rows(one, p_lo, b_lo - 1, 1);
+      else sweep_rows(one, m_lo, b_lo - 1, -1);
+      sweep_rows(two, b_lo, b_hi, 0);
+      if (p_hi > m_hi) sweep_rows(one, b_hi + 1, p_hi, 1);
+      else sweep_rows(one, b_hi + 1, m_hi, -1);
+    }
+  } else {
+    for (int dk = 0; dk <= kmax; ++dk) {
+      const float dyv = (float)dk * a.dy;
+      const float r2 = __builtin_fmaf(dyv, dyv, d2);
+      const int kp = kj + dk, km = kj - dk;
+      const bool has_kp = kp >= xd_lo && kp < xd_hi;
+      const bool has_km = dk > 0 && km >= xd_lo && km < xd_hi;
+      if (!has_kp && !has_km) continue;
+      const bool in_mask = r2 <= p.R2;
+      const float tau = sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq));
+      const float q = tau * a.inv_dt;
+      const int i = (int)q;
+      bool okp = true, okm = true;
+      if (WINDOW) {
+        okp = kp >= p.wlo && kp < p.whi;
+        okm = km >= p.wlo && km < p.whi;
+      }
+      if constexpr (COUNT) {
+        const unsigned nrow = (hp ? 1u : 0u) + (hm ? 1u : 0u);
+        const unsigned n = nrow * ((has_kp && okp ? 1u : 0u) + (has_km && okm ? 1u : 0u));
+        cap += in_mask ? n : 0u;
+        cg += (in_mask && i < nt1) ? n : 0u;
+      } else if constexpr (MODE == kAdjoint) {
+        // d[b][k][i] += w (1 - f) m,  d[b][k][i+1] += w f m  (transpose of
+        // the interpolation fma(f, x[i+1] - x[i], x[i]))
+        if (in_mask && i < nt1) {
           const float f = __builtin_amdgcn_fractf(q);
-          const typename S::Raw x0 = S::load_ptr(rowp + (size_t)xp_ * tb + off);
-          const typename S::Raw x1 = S::load_ptr(rowp + (size_t)xm_ * tb + off);
-          const typename S::Raw x2 = S::load_ptr(rowm + (size_t)xp_ * tb + off);
-          const typename S::Raw x3 = S::load_ptr(rowm + (size_t)xm_ * tb + off);
-          float l0 = S::lerp(x0, f);
-          float l1 = S::lerp(x1, f);
-          float l2 = S::lerp(x2, f);
-          float l3 = S::lerp(x3, f);
-          if constexpr (S::kScaled) {
-            l0 *= scp[xp_];
-            l1 *= scp[xm_];
-            l2 *= scm[xp_];
-            l3 *= scm[xm_];
-          }
-          l0 = okp ? l0 : 0.0f;
-          l2 = okp ? l2 : 0.0f;
-          l1 = okm ? l1 : 0.0f;
-          l3 = okm ? l3 : 0.0f;
-          float sum = (l0 + l1) + (l2 + l3);
-          sum = in_mask ? sum : 0.0f;
           const float w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
-          if constexpr (MODE == kTangent) {
-            // d/d(alpha) of w * lerp(x, tau/dt) with v -> alpha v and the mask
-            // held fixed: dtau = -(a r^2) / (alpha tau), dq = dtau / dt,
-            // dw = -w (dtau / tau + 2 / alpha); d lerp / dq = x[i+1] - x[i]
-            float dsum = 0.0f;
-            if constexpr (!S::kScaled) {
-              dsum = (okp ? x0.y + x2.y : 0.0f) + (okm ? x1.y + x3.y : 0.0f);
+          const float wm = w * p.mval;
+          const float c0 = wm - wm * f, c1 = wm * f;
+          auto scatter = [&](bool row_ok, int row, bool k_ok, int k) {
+            if (row_ok && k_ok) {
+              float* d = a.adj_out + ((size_t)row * a.n_xd + (k - xd_lo)) * n_t + i;
+              unsafeAtomicAdd(d, c0);
+              unsafeAtomicAdd(d + 1, c1);
             }
-            dsum = in_mask ? dsum : 0.0f;
-            const float dtau = -(p.a * r2) / (a.alpha * tau);
-            const float dq = dtau * a.inv_dt;
-            const float dw = a.amp ? -w * (dtau / tau + 2.0f / a.alpha) : 0.0f;
-            acc += dw * sum + w * dq * dsum;
-          } else {
-            acc = __builtin_fmaf(sum, w, acc);
+          };
+          scatter(hp, ip, has_kp && okp, kp);
+          scatter(hp, ip, has_km && okm, km);
+          scatter(hm, im, has_kp && okp, kp);
+          scatter(hm, im, has_km && okm, km);
+        }
+      } else {
+        const int xp_ = has_kp ? kp - xd_lo : a.n_xd;
+        const int xm_ = has_km ? km - xd_lo : a.n_xd;
+        const size_t off = (size_t)min(i, nt1) * S::kBytes;
+        const float f = __builtin_amdgcn_fractf(q);
+        const typename S::Raw x0 = S::load_ptr(rowp + (size_t)xp_ * tb + off);
+        const typename S::Raw x1 = S::load_ptr(rowp + (size_t)xm_ * tb + off);
+        const typename S::Raw x2 = S::load_ptr(rowm + (size_t)xp_ * tb + off);
+        const typename S::Raw x3 = S::load_ptr(rowm + (size_t)xm_ * tb + off);
+        float l0 = S::lerp(x0, f);
+        float l1 = S::lerp(x1, f);
+        float l2 = S::lerp(x2, f);
+        float l3 = S::lerp(x3, f);
+        if constexpr (S::kScaled) {
+          l0 *= scp[xp_];
+          l1 *= scp[xm_];
+          l2 *= scm[xp_];
+          l3 *= scm[xm_];
+        }
+        l0 = okp ? l0 : 0.0f;
+        l2 = okp ? l2 : 0.0f;
+        l1 = okm ? l1 : 0.0f;
+        l3 = okm ? l3 : 0.0f;
+        float sum = (l0 + l1) + (l2 + l3);
+        sum = in_mask ? sum : 0.0f;
+        const float w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+        if constexpr (MODE == kTangent) {
+          // d/d(alpha) of w * lerp(x, tau/dt) with v -> alpha v and the mask
+          // held fixed: dtau = -(a r^2) / (alpha tau), dq = dtau / dt,
+          // dw = -w (dtau / tau + 2 / alpha); d lerp / dq = x[i+1] - x[i]
+          float dsum = 0.0f;
+          if constexpr (!S::kScaled) {
+            dsum = (okp ? x0.y + x2.y : 0.0f) + (okm ? x1.y + x3.y : 0.0f);
           }
+          dsum = in_mask ? dsum : 0.0f;
+          const float dtau = -(p.a * r2) / (a.alpha * tau);
+          const float dq = dtau * a.inv_dt;
+          const float dw = a.amp ? -w * (dtau / tau + 2.0f / a.alpha) : 0.0f;
+          acc += dw * sum + w * dq * dsum;
+        } else {
+          acc = __builtin_fmaf(sum, w, acc);
         }
       }
     }
+  }
+}
+
+// Sums the aperture of one (output trace, t0 tile) over the row offsets
+// |db| in [d_lo, d_hi) of this pass (generic rows: the data rows
+// [d_lo, d_hi)), one sweep_row per row pair.
+template <int ST, bool GENERIC_ROWS, bool WINDOW, int MODE>
+__device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, int kj,
+                                             const LaneParams& p, float R2max,
+                                             float& acc, unsigned& cap, unsigned& cg) {
+  using S = Store<ST>;
+  const int tb = a.n_t * S::kBytes;
+  const size_t row_bytes = (size_t)(a.n_xd + 1) * tb;
+  const char* const base = reinterpret_cast<const char*>(a.xp);
+  const char* const zero_row = base + (size_t)a.n_rows * row_bytes;
+  auto row_sweep = [&](const char* rowp, bool hp, int ip, const char* rowm, bool hm, int im,
+                       float d2) {
+    sweep_row<ST, WINDOW, MODE>(a, kj, p, R2max, acc, cap, cg, rowp, hp, ip, rowm, hm, im, d2);
   };
+
 
   if constexpr (GENERIC_ROWS) {
     // generic rows: this pass covers the data rows [d_lo, d_hi)
@@ -441,23 +480,18 @@ __device__ __forceinline__ void sum_aperture(const MigArgs& a, const Task& tk, i
   }
 }
 
-// The work of one wave: output trace (io, j) of the output inline io, t0 tile
-// `tile` (64 samples).
-template <int ST, bool GENERIC_ROWS, int MODE>
-__device__ __forceinline__ void migrate_trace_tile(const MigArgs& a, int tile, int io, int j) {
-  constexpr bool COUNT = MODE == kCount;
-  constexpr bool WRITES_OUT = MODE == kForward || MODE == kTangent;
-  const int lane = threadIdx.x % kWave;
-  const int s = tile * kWave + lane;
-  const bool active = s < a.n_t;
-  const int kj = a.cur_xl_start + j;
-  float* orow = a.out + ((size_t)io * a.n_out_xl + j) * a.n_t;
+// Image-point parameters of one lane (sample s of output trace (io, j),
+// buffer crossline kj in [0, K)) and its sweep radius.
+struct LaneSetup {
+  LaneParams p;
+  int src;            // crossline whose velocity the lane uses; -1: the sample is not valid
+  bool needs_window;  // src != kj and its crossline window can cut the aperture
+  float L;            // last useful r^2 of this lane (-1: none)
+};
 
-  if (kj < 0 || kj >= a.K) {  // migration_kernel.cpp:79-85
-    if (WRITES_OUT && !a.accumulate && active) orow[s] = 0.0f;
-    return;
-  }
-  const Task tk = a.tasks[io];
+template <int MODE>
+__device__ __forceinline__ LaneSetup lane_setup(const MigArgs& a, const Task& tk, int io, int j,
+                                                int kj, int s, bool active) {
   const float* vrowp = a.vel + (size_t)tk.vrow * a.n_xv * a.n_t;
 
   // ---- parameter source: this crossline, or the last earlier crossline of
@@ -515,6 +549,31 @@ __device__ __forceinline__ void migrate_trace_tile(const MigArgs& a, int tile, i
       L = fminf(L, (float)(r2_end * (1.0 + 1e-6)));
     }
   }
+  return LaneSetup{p, src, needs_window, L};
+}
+
+// The work of one wave: output trace (io, j) of the output inline io, t0 tile
+// `tile` (64 samples).
+template <int ST, bool GENERIC_ROWS, int MODE>
+__device__ __forceinline__ void migrate_trace_tile(const MigArgs& a, int tile, int io, int j) {
+  constexpr bool COUNT = MODE == kCount;
+  constexpr bool WRITES_OUT = MODE == kForward || MODE == kTangent;
+  const int lane = threadIdx.x % kWave;
+  const int s = tile * kWave + lane;
+  const bool active = s < a.n_t;
+  const int kj = a.cur_xl_start + j;
+  float* orow = a.out + ((size_t)io * a.n_out_xl + j) * a.n_t;
+
+  if (kj < 0 || kj >= a.K) {  // migration_kernel.cpp:79-85
+    if (WRITES_OUT && !a.accumulate && active) orow[s] = 0.0f;
+    return;
+  }
+  const Task tk = a.tasks[io];
+  const LaneSetup ls = lane_setup<MODE>(a, tk, io, j, kj, s, active);
+  const LaneParams& p = ls.p;
+  const int src = ls.src;
+  const bool needs_window = ls.needs_window;
+  const float L = ls.L;
   const float R2max = uniform_f(wave_max(L));
   float acc = 0.0f;
   unsigned cap = 0, cg = 0;
@@ -554,6 +613,431 @@ __global__ __launch_bounds__(kWave * WPB) void migrate_kernel(MigArgs a, int n_t
   const int j = (spatial % n_jgroups) * WPB + uniform_i((int)(threadIdx.x / kWave));
   if (j >= a.n_out_xl) return;
   migrate_trace_tile<ST, GENERIC_ROWS, MODE>(a, tile, io, j);
+}
+
+// ---------------------------------------------------------------- LDS-staged forward path
+// A workgroup of W waves migrates the W adjacent output crosslines
+// j0 .. j0+W-1 of one output inline over one t0 tile.  For each aperture row
+// pair (+db, -db) the crossline offsets dk are swept in groups of G steps.
+// Wave w at step dk reads the columns kj_w + dk ("side 0") and kj_w - dk
+// ("side 1"), i.e. the column offsets o = w + dk on side 0 (column j0 + o)
+// and o = W-1-w + dk on side 1 (column j0 + W-1 - o): all waves walk the same
+// two column sequences, and each column is read by up to W waves at W
+// consecutive steps.  The stored pairs any of those reads can touch lie in a
+// window [lo, hi] bounded from the waves' smallest and largest t0^2 and 4/v^2
+// (q = sqrt(t0^2 + a r^2)/dt is nondecreasing in both); the window is copied
+// once into LDS (16 bytes per lane, one coalesced load per window) into a ring
+// of slots per (row, side) that is filled one group ahead of the reads.  The
+// waves gather from LDS instead of through the texture path and keep the
+// four-partner sharing of tau, i, f and the weight; a column outside the data
+// is staged as zeros.  A row whose windows exceed a slot, or whose column
+// offsets exceed the column table, runs the gather path (sweep_row); so does
+// a workgroup with lanes whose sticky crossline window can cut the aperture.
+constexpr int kWinPairs = 128;  // stored pairs per staged window
+constexpr int kMaxCols = 512;   // column offsets per side one row can stage
+
+// One staging job (column offset o, side, ring row rr; ok = the job exists)
+// and the table values of one group of steps (r^2 and the two column
+// addresses per step, the column table entries of the staging jobs).
+struct StageJob {
+  int o, side, rr;
+  bool ok;
+};
+template <int G, int J>
+struct StageTab {
+  float r2[G];
+  int bR[G], bL[G];
+  int mj[J];
+};
+
+template <int ST, int W, int G>
+struct LdsPlan {
+  static constexpr int kB = Store<ST>::kBytes;
+  static constexpr int kSlot = kWinPairs * kB;           // bytes of one window
+  static constexpr int kRing = 2 * G + W - 1;            // slots per (ring row, side)
+  static constexpr int kSide = kRing * kSlot;            // side 0 -> side 1
+  static constexpr int kRow = 2 * kSide;                 // ring row 0 -> ring row 1
+  static constexpr int kMeta = 2 * kRow;                 // int meta[2][kMaxCols]
+  static constexpr int kR2 = kMeta + 2 * kMaxCols * 4;   // float r2tab[kMaxCols]
+  static constexpr int kPar = kR2 + kMaxCols * 4;        // float par[5][W]
+  static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
+  static constexpr int kBytes = kFlag + W * 4;
+  static_assert(kRow <= 65535, "ring row offset must fit a ds_read offset");
+  static_assert(kSlot % (16 * 1) == 0 && kSlot <= 16 * kWave, "one load per window");
+  __device__ static int slot(int side, int o) { return side * kSide + (o % kRing) * kSlot; }
+};
+
+// Bytes a row of the pair layout may span for the staged path (the row is
+// addressed through a buffer resource with a 32-bit size).
+constexpr size_t kLdsMaxRowBytes = (size_t)1 << 30;
+
+template <int ST, int W, int G>
+__global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n_tiles,
+                                                                int n_jgroups, int per_tile_pad) {
+  using S = Store<ST>;
+  using Raw = typename S::Raw;
+  using PL = LdsPlan<ST, W, G>;
+  constexpr int kB = S::kBytes;
+  __shared__ __attribute__((aligned(16))) char smem[PL::kBytes];
+
+  // ---- block -> (t0 tile, output inline, crossline group), as migrate_kernel
+  const int g = blockIdx.x;
+  const int tile = n_tiles - 1 - g / per_tile_pad;
+  const int rem = g % per_tile_pad;
+  const int spatial = (rem % kXcds) * (per_tile_pad / kXcds) + rem / kXcds;
+  if (spatial >= a.n_out_il * n_jgroups) return;  // whole workgroup
+  const int io = spatial / n_jgroups;
+  const int j0 = (spatial % n_jgroups) * W;
+  const int w = uniform_i((int)(threadIdx.x / kWave));
+  const int lane = threadIdx.x % kWave;
+  const int j = j0 + w;
+  const int kj = a.cur_xl_start + j;
+  const int s = tile * kWave + lane;
+  const bool active = s < a.n_t;
+  const bool in_out = j < a.n_out_xl;
+  const bool in_buf = kj >= 0 && kj < a.K;
+  const Task tk = a.tasks[io];
+
+  LaneSetup ls{{0.0f, 0.0f, -1.0f, 1.0f, 0, 0, 0.0f}, -1, false, -1.0f};
+  if (in_out && in_buf) ls = lane_setup<kForward>(a, tk, io, j, kj, s, active);
+  const bool valid = ls.src >= 0;
+  const float R2w = uniform_f(wave_max(ls.L));
+  const float Tlo2 = uniform_f(wave_min(valid ? ls.p.t0sq : 3.0e38f));
+  const float Thi2 = uniform_f(wave_max(valid ? ls.p.t0sq : 0.0f));
+  const float Alo = uniform_f(wave_min(valid ? ls.p.a : 3.0e38f));
+  const float Ahi = uniform_f(wave_max(valid ? ls.p.a : 0.0f));
+  const bool win = __any(ls.needs_window);
+
+  int* const meta = reinterpret_cast<int*>(smem + PL::kMeta);
+  float* const r2tab = reinterpret_cast<float*>(smem + PL::kR2);
+  float* const par = reinterpret_cast<float*>(smem + PL::kPar);
+  int* const flag = reinterpret_cast<int*>(smem + PL::kFlag);
+  if (lane == 0) {
+    par[0 * W + w] = R2w;
+    par[1 * W + w] = Tlo2;
+    par[2 * W + w] = Thi2;
+    par[3 * W + w] = Alo;
+    par[4 * W + w] = Ahi;
+    flag[w] = win ? 1 : 0;
+  }
+  __syncthreads();
+  // lane l < W holds the parameters of wave l
+  const bool pl = lane < W;
+  const float pR2 = pl ? par[lane] : -1.0f;
+  const float pTlo2 = pl ? par[W + lane] : 0.0f;
+  const float pThi2 = pl ? par[2 * W + lane] : 0.0f;
+  const float pAlo = pl ? par[3 * W + lane] : 0.0f;
+  const float pAhi = pl ? par[4 * W + lane] : 0.0f;
+  const bool any_win = __any(pl && flag[pl ? lane : 0] != 0);
+  const float R2wg = uniform_f(wave_max(pR2));
+  // workgroup bounds of t0^2 and 4/v^2 over the valid lanes of all waves
+  const float gTlo2 = uniform_f(wave_min(pl ? pTlo2 : 3.0e38f));
+  const float gThi2 = uniform_f(wave_max(pl ? pThi2 : 0.0f));
+  const float gAlo = uniform_f(wave_min(pl ? pAlo : 3.0e38f));
+  const float gAhi = uniform_f(wave_max(pl ? pAhi : 0.0f));
+
+  float acc = 0.0f;
+  unsigned cap = 0, cg = 0;
+  if (any_win) {
+    // sticky crossline windows in the group: every wave runs the gather path
+    if (R2w >= 0.0f) {
+      if (win)
+        sum_aperture<ST, false, true, kForward>(a, tk, kj, ls.p, R2w, acc, cap, cg);
+      else
+        sum_aperture<ST, false, false, kForward>(a, tk, kj, ls.p, R2w, acc, cap, cg);
+    }
+  } else if (R2wg >= 0.0f) {
+    const int c = tk.c;
+    const int nt1 = a.n_t - 1;
+    const int xd_lo = a.xd_off, xd_hi = a.xd_off + a.n_xd;
+    const int tb = a.n_t * kB;
+    const size_t row_bytes = (size_t)(a.n_xd + 1) * tb;
+    const char* const base = reinterpret_cast<const char*>(a.xp);
+    const char* const zero_row = base + (size_t)a.n_rows * row_bytes;
+    const int row_tr = a.n_xd + 1;
+    const int kbase = a.cur_xl_start + j0;  // buffer crossline of side-0 column offset 0
+    // A lane without a valid sample reads at the lower corner of its wave's
+    // bounds, inside every window it touches; its terms are masked (R2 = -1).
+    LaneParams p = ls.p;
+    if (!valid) {
+      p.t0sq = Tlo2;
+      p.a = Alo;
+    }
+
+    int dmax = (int)(__builtin_sqrtf(R2wg) / a.dx) + 2;
+    dmax = min(dmax, max(tk.b1 - 1 - c, c - tk.b0));
+    dmax = uniform_i(min(dmax, a.d_hi - 1));
+    for (int d = max(0, a.d_lo); d <= dmax; ++d) {
+      const float dxv = (float)d * a.dx;
+      const float d2 = uniform_f(dxv * dxv);
+      if (d2 > R2wg) break;
+      const int rp = c + d, rm = c - d;
+      const bool hp = rp >= tk.b0 && rp < tk.b1;
+      const bool hm = d > 0 && rm >= tk.b0 && rm < tk.b1;
+      if (!hp && !hm) continue;
+      // last crossline offset of each wave's sweep (lane l < W: wave l), as
+      // in sweep_row: r^2 <= R2max, clipped to offsets that can hold data
+      int km = -1;
+      if (pl && pR2 >= d2) {
+        const int kjl = kbase + lane;
+        int k = (int)(__builtin_sqrtf(pR2 - d2) * a.inv_dy) + 2;
+        k = min(k, max(kjl - xd_lo, xd_hi - 1 - kjl));
+        while (k >= 0) {
+          const float dyv = (float)k * a.dy;
+          if (__builtin_fmaf(dyv, dyv, d2) <= pR2) break;
+          --k;
+        }
+        km = k;
+      }
+      const int kmax_wg = uniform_i(wave_max_i(km));
+      if (kmax_wg < 0) continue;
+      const int kmax_w = __builtin_amdgcn_readlane(km, w);
+      const int nO = kmax_wg + W;  // column offsets per side
+      const int r0 = hp ? rp : rm;  // ring row 0; ring row 1 = rm when both rows exist
+      const bool two = hp && hm;
+
+      bool fallback = nO > kMaxCols;
+      if (!fallback) {
+        __syncthreads();  // the previous row's windows and tables are no longer read
+        // ---- column tables: meta[side][o] = LDS address of stored pair 0 of
+        // the column's window (slot - lo * kB); this wave fills o = w (mod W).
+        // Column o of side s is read at the offsets dk in [max(s, o-W+1),
+        // min(o, kmax)] (one per wave), so q of every read lies between
+        // sqrt(t0lo^2 + alo r^2(dk_lo)) / dt and sqrt(t0hi^2 + ahi r^2(dk_hi)) / dt
+        // with the workgroup's bounds of t0^2 and a = 4/v^2.  r2tab[dk] =
+        // r^2 of the row pair at crossline offset dk.
+        bool ovf = false;
+        for (int ob = w; ob < nO; ob += W * kWave) {
+          const int o = ob + W * lane;
+          if (o < nO) {
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+              const int dlo = max(side, o - (W - 1)), dhi = min(o, kmax_wg);
+              int lo = 0;
+              if (dlo <= dhi) {
+                const float ylo = (float)dlo * a.dy, yhi = (float)dhi * a.dy;
+                const float lo_t = __builtin_sqrtf(
+                    __builtin_fmaf(gAlo, __builtin_fmaf(ylo, ylo, d2), gTlo2));
+                const float hi_t = __builtin_sqrtf(
+                    __builtin_fmaf(gAhi, __builtin_fmaf(yhi, yhi, d2), gThi2));
+                // one index of slack each way covers the rounding of q
+                lo = min(max((int)(lo_t * a.inv_dt) - 1, 0), nt1);
+                const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
+                ovf = ovf || (hi - lo >= kWinPairs);
+              }
+              meta[side * kMaxCols + o] = PL::slot(side, o) - lo * kB;
+            }
+          }
+        }
+        for (int dk = w * kWave + lane; dk <= kmax_wg; dk += W * kWave) {
+          const float dyv = (float)dk * a.dy;
+          r2tab[dk] = __builtin_fmaf(dyv, dyv, d2);
+        }
+        if (lane == 0) flag[w] = __any(ovf) ? 1 : 0;
+        __syncthreads();  // tables and flags are visible
+        fallback = __any(pl && flag[pl ? lane : 0] != 0);
+      }
+      if (!fallback) {
+        // ---- staging jobs: job j of a set of columns [oa, oa + nc) is
+        // (column oa + j / (2 nr), side (j / nr) % 2, ring row j % nr), nr =
+        // rows of the pair; wave w takes jobs w, w + W, ...  A job loads the
+        // column's window of one row into registers, 16 bytes per lane
+        // (load_job), and writes it to the column's ring slot (store_job), so
+        // LDS reads never wait on loads in flight.  Columns outside the data
+        // and samples past the trace end are loaded out of range (zeros).
+        using V4 = unsigned __attribute__((ext_vector_type(4)));
+        const int nr = two ? 2 : 1;
+        const int lb = lane * 16;
+        const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char*>(base + (size_t)r0 * row_bytes), 0, (int)row_bytes, kRsrcWord3);
+        const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char*>(base + (size_t)rm * row_bytes), 0, (int)row_bytes, kRsrcWord3);
+        // A job's source: column o, side, ring row rr of the job index j
+        // within the columns [oa, oa + nc).
+        auto job_at = [&](int oa, int nc, int j) {
+          return StageJob{oa + j / (2 * nr), (j / nr) % 2, j % nr, j < nc * 2 * nr};
+        };
+        // buffer offsets of a job whose table entry is m (meta of its column)
+        auto job_src = [&](const StageJob& jb, int m, int& voff, int& soff) {
+          const int k = jb.side == 0 ? kbase + jb.o : kbase + (W - 1) - jb.o;
+          const bool data = k >= xd_lo && k < xd_hi;
+          const int lo = uniform_i((PL::slot(jb.side, jb.o) - m) / kB);
+          soff = data ? (k - xd_lo) * tb : 0;
+          const bool zero = !data || lo * kB + lb > nt1 * kB;
+          voff = zero ? 0x7ffffff0 : lo * kB + lb;
+        };
+        auto job_dst = [&](const StageJob& jb) { return PL::slot(jb.side, jb.o) + jb.rr * PL::kRow; };
+        auto tab = [&](int side, int o) { return meta[side * kMaxCols + min(o, kMaxCols - 1)]; };
+        {  // the first G + W - 1 columns, loaded straight into LDS (no
+           // registers held); complete before the barrier
+          constexpr int kFill = (4 * (G + W - 1) + W - 1) / W;
+          const int nc = min(G + W - 1, nO);
+          int m[kFill];
+#pragma unroll
+          for (int t = 0; t < kFill; ++t) {
+            const StageJob jb = job_at(0, nc, w + t * W);
+            m[t] = jb.ok ? tab(jb.side, jb.o) : 0;
+          }
+#pragma unroll
+          for (int t = 0; t < kFill; ++t) {
+            const StageJob jb = job_at(0, nc, w + t * W);
+            if (jb.ok && lb < PL::kSlot) {
+              int voff, soff;
+              job_src(jb, m[t], voff, soff);
+              __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                  jb.rr == 0 ? rs0 : rs1,
+                  (__attribute__((address_space(3))) void*)(smem + job_dst(jb)), 16, voff, soff,
+                  0, 0);
+            }
+          }
+          __builtin_amdgcn_s_waitcnt(0);
+        }
+        // scale tables of the staged rows (F16 / BF16)
+        const const_f32_ptr sc0 = as_const(a.scale + (size_t)r0 * row_tr);
+        const const_f32_ptr sc1 = as_const(a.scale + (size_t)rm * row_tr);
+        // Table values of one group, read one group ahead so that their LDS
+        // latency overlaps the stores and the barrier: r^2, the two column
+        // addresses of each step, and the table entries of the staging jobs
+        // for the group after it.
+        constexpr int kJobs = (4 * G + W - 1) / W;  // staging jobs per wave per group, at most
+        using Tab = StageTab<G, kJobs>;
+        auto read_tab = [&](int dk0, Tab& t) {
+#pragma unroll
+          for (int u = 0; u < G; ++u) {
+            const int dk = min(dk0 + u, kMaxCols - W);
+            t.r2[u] = r2tab[dk];
+            t.bR[u] = tab(0, w + dk);
+            t.bL[u] = tab(1, (W - 1 - w) + dk);
+          }
+          const int oa = dk0 + G + W - 1;
+          const int nc = max(0, min(G, nO - oa));
+#pragma unroll
+          for (int q = 0; q < kJobs; ++q) {
+            const StageJob jb = job_at(oa, nc, w + q * W);
+            t.mj[q] = jb.ok ? tab(jb.side, jb.o) : 0;
+          }
+        };
+        auto group = [&](auto nr_c, int dk0, int dk_end, const Tab& t) {
+          constexpr int NR = decltype(nr_c)::value;
+          struct Stp {
+            float f, wt;
+            bool m;
+            Raw x[2 * NR];
+            float sc[2 * NR];
+          };
+          Stp st[G];
+#pragma unroll
+          for (int u = 0; u < G; ++u) {
+            const int dk = dk0 + u;
+            if (dk < dk_end) {
+              const float r2 = t.r2[u];
+              const float tau = sqrt_up(__builtin_fmaf(p.a, r2, p.t0sq));
+              const float q = tau * a.inv_dt;
+              const int ib = min((int)q, nt1) * kB;
+              const char* xr = smem + t.bR[u] + ib;
+              const char* xl = smem + t.bL[u] + ib;
+              st[u].x[0] = *reinterpret_cast<const Raw*>(xr);
+              st[u].x[1] = dk > 0 ? *reinterpret_cast<const Raw*>(xl) : Raw{};
+              if constexpr (NR == 2) {
+                st[u].x[2] = *reinterpret_cast<const Raw*>(xr + PL::kRow);
+                st[u].x[3] = dk > 0 ? *reinterpret_cast<const Raw*>(xl + PL::kRow) : Raw{};
+              }
+              if constexpr (S::kScaled) {
+                int cR = kj + dk - xd_lo, cL = kj - dk - xd_lo;
+                cR = (unsigned)cR < (unsigned)a.n_xd ? cR : a.n_xd;
+                cL = (dk > 0 && (unsigned)cL < (unsigned)a.n_xd) ? cL : a.n_xd;
+                st[u].sc[0] = sc0[cR];
+                st[u].sc[1] = sc0[cL];
+                if constexpr (NR == 2) {
+                  st[u].sc[2] = sc1[cR];
+                  st[u].sc[3] = sc1[cL];
+                }
+              }
+              st[u].f = __builtin_amdgcn_fractf(q);
+              st[u].wt = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+              st[u].m = r2 <= p.R2;
+            }
+          }
+          // slot order (row, column) = (+db,+dk), (+db,-dk), (-db,+dk),
+          // (-db,-dk); the -dk column of dk = 0 holds zeros
+#pragma unroll
+          for (int u = 0; u < G; ++u) {
+            if (dk0 + u < dk_end) {
+              const Stp& q = st[u];
+              float l[2 * NR];
+#pragma unroll
+              for (int k = 0; k < 2 * NR; ++k) l[k] = S::lerp(q.x[k], q.f);
+              float sum;
+              if constexpr (S::kScaled) {
+                sum = l[0] * q.sc[0];
+#pragma unroll
+                for (int k = 1; k < 2 * NR; ++k) sum = __builtin_fmaf(l[k], q.sc[k], sum);
+              } else if constexpr (NR == 2) {
+                sum = (l[0] + l[1]) + (l[2] + l[3]);
+              } else {
+                sum = l[0] + l[1];
+              }
+              sum = q.m ? sum : 0.0f;
+              acc = __builtin_fmaf(sum, q.wt, acc);
+            }
+          }
+        };
+        Tab cur;
+        read_tab(0, cur);
+        __syncthreads();  // the first windows are visible
+        for (int dk0 = 0; dk0 <= kmax_wg; dk0 += G) {
+          // the next group's G new columns, in flight while this group runs;
+          // their slots are not read by this group (ring of 2G + W - 1 slots)
+          const int oa = dk0 + G + W - 1;
+          const int nc = max(0, min(G, nO - oa));
+          // every wave issues the same loads whether or not its jobs exist
+          // (absent jobs read out of range), so no loaded value is merged
+          // across a branch and nothing waits on the loads before the stores
+          V4 v[kJobs];
+#pragma unroll
+          for (int q = 0; q < kJobs; ++q) {
+            const StageJob jb = job_at(oa, nc, w + q * W);
+            int voff, soff;
+            job_src(jb, cur.mj[q], voff, soff);
+            v[q] = __builtin_amdgcn_raw_buffer_load_b128(jb.rr == 0 ? rs0 : rs1,
+                                                         jb.ok ? voff : 0x7ffffff0, soff, 0);
+          }
+          const int dk_end = min(dk0 + G, kmax_w + 1);
+          if (dk0 < dk_end) {
+            if (two)
+              group(std::integral_constant<int, 2>{}, dk0, dk_end, cur);
+            else
+              group(std::integral_constant<int, 1>{}, dk0, dk_end, cur);
+          }
+          read_tab(dk0 + G, cur);
+#pragma unroll
+          for (int q = 0; q < kJobs; ++q) {
+            const StageJob jb = job_at(oa, nc, w + q * W);
+            if (jb.ok && lb < PL::kSlot) *reinterpret_cast<V4*>(smem + job_dst(jb) + lb) = v[q];
+          }
+          __syncthreads();  // the next group's windows are visible
+        }
+        continue;
+      }
+      // gather path for this row
+      if (R2w >= d2)
+        sweep_row<ST, false, kForward>(a, kj, p, R2w, acc, cap, cg,
+                                       hp ? base + (size_t)rp * row_bytes : zero_row, hp,
+                                       hp ? rp : a.n_rows,
+                                       hm ? base + (size_t)rm * row_bytes : zero_row, hm,
+                                       hm ? rm : a.n_rows, d2);
+    }
+  }
+  if (in_out && active) {
+    float* orow = a.out + ((size_t)io * a.n_out_xl + j) * a.n_t;
+    if (!in_buf) {
+      if (!a.accumulate) orow[s] = 0.0f;  // migration_kernel.cpp:79-85
+    } else if (a.accumulate) {
+      if (valid) orow[s] += acc;
+    } else {
+      orow[s] = valid ? acc : 0.0f;  // :148-151, :375
+    }
+  }
 }
 
 // Persistent batch kernel: every workgroup takes work items (one
@@ -771,6 +1255,26 @@ hipError_t launch_wpb(const MigArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+template <int ST, int W, int G>
+hipError_t launch_lds_wg(const MigArgs& a, hipStream_t stream) {
+  const int n_tiles = (a.n_t + kWave - 1) / kWave;
+  const int n_jgroups = (a.n_out_xl + W - 1) / W;
+  const long long per_tile = (long long)a.n_out_il * n_jgroups;
+  const long long per_tile_pad = (per_tile + kXcds - 1) / kXcds * kXcds;
+  const long long blocks = per_tile_pad * n_tiles;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((migrate_lds_kernel<ST, W, G>), dim3((unsigned)blocks), dim3(kWave * W), 0,
+                     stream, a, n_tiles, n_jgroups, (int)per_tile_pad);
+  return hipGetLastError();
+}
+
+template <int ST>
+hipError_t launch_lds_st(const MigArgs& a, hipStream_t stream, int wpb, int steps) {
+  if (wpb == 8)
+    return steps == 2 ? launch_lds_wg<ST, 8, 2>(a, stream) : launch_lds_wg<ST, 8, 4>(a, stream);
+  return steps == 2 ? launch_lds_wg<ST, 4, 2>(a, stream) : launch_lds_wg<ST, 4, 4>(a, stream);
+}
+
 template <int ST, int MODE>
 hipError_t launch_st(const MigArgs& a, hipStream_t stream, int wpb) {
   switch (wpb) {
@@ -789,9 +1293,18 @@ hipError_t launch_migrate(const MigArgs& a0, int mode, hipStream_t stream, const
   // aperture-row passes: pass p sums the row offsets [p*R, (p+1)*R) (generic
   // rows: row indices); passes after the first accumulate into `out`
   const bool f32_traces = mode != kForward || a0.storage == PSM_STORE_F32;
-  const int rpp = tune.rows_per_pass >= 0 ? tune.rows_per_pass : (f32_traces ? kAutoRowsF32 : 0);
+  // LDS-staged forward kernel: contiguous rows, 4 or 8 output crosslines per
+  // workgroup, rows addressable by a 32-bit buffer size
+  const size_t row_bytes = (size_t)(a0.n_xd + 1) * a0.n_t * (a0.storage == PSM_STORE_F32 ? 8 : 4);
+  const bool lds_ok = mode == kForward && a0.row_pos == nullptr && row_bytes <= kLdsMaxRowBytes;
+  const bool auto_lds = tune.staging < 0 && a0.storage == PSM_STORE_F32 && lds_ok;
+  const int wpb = auto_lds ? kAutoStageWaves : tune.waves_per_block;
+  const int steps = auto_lds ? kAutoStageSteps : tune.stage_steps;
+  const bool lds = auto_lds || (tune.staging > 0 && lds_ok && (wpb == 4 || wpb == 8));
+  // automatic passes: 8 row offsets per launch for FP32 gathers, one launch otherwise
+  const int rpp = tune.rows_per_pass >= 0 ? tune.rows_per_pass
+                                          : ((f32_traces && !lds) ? kAutoRowsF32 : 0);
   const int R = (mode == kCount || rpp <= 0) ? (row_extent + 1) : rpp;
-  const int wpb = tune.waves_per_block;
   for (int lo = 0, pass = 0; lo <= row_extent; lo += R, ++pass) {
     a.d_lo = lo;
     a.d_hi = lo + R;
@@ -802,6 +1315,14 @@ hipError_t launch_migrate(const MigArgs& a0, int mode, hipStream_t stream, const
       case kAdjoint: e = launch_st<PSM_STORE_F32, kAdjoint>(a, stream, wpb); break;
       case kTangent: e = launch_st<PSM_STORE_F32, kTangent>(a, stream, wpb); break;
       default:
+        if (lds) {
+          switch (a.storage) {
+            case PSM_STORE_F16: e = launch_lds_st<PSM_STORE_F16>(a, stream, wpb, steps); break;
+            case PSM_STORE_BF16: e = launch_lds_st<PSM_STORE_BF16>(a, stream, wpb, steps); break;
+            default: e = launch_lds_st<PSM_STORE_F32>(a, stream, wpb, steps); break;
+          }
+          break;
+        }
         switch (a.storage) {
           case PSM_STORE_F16: e = launch_st<PSM_STORE_F16, kForward>(a, stream, wpb); break;
           case PSM_STORE_BF16: e = launch_st<PSM_STORE_BF16, kForward>(a, stream, wpb); break;

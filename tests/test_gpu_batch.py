@@ -1,7 +1,8 @@
 """Batched, graph-captured persistent migration: a velocity scan over scaled
 Vrms models plus uneven sub-volumes in one launch.  Each output must equal
-the single-volume migration of that item at the same velocity scale (the same
-per-wave arithmetic, so bit for bit), on every replay of the graph with eager
+the single-volume migration of that item at the same velocity scale by the
+gather form of the forward kernel in one launch (the same per-wave arithmetic
+and summation order, so bit for bit), on every replay of the graph with eager
 work in between, and after autotuning."""
 import numpy as np
 import pytest
@@ -39,6 +40,7 @@ def _reference(items, storage, wpb):
     try:
         ctx.set_storage(storage)
         ctx.set_tuning(wpb, 0)
+        ctx.set_staging(False)
         refs = []
         for prm, d, v, out, alpha in items:
             ctx.set_velocity_scale(alpha, alpha)

@@ -216,8 +216,13 @@ def test_volume_launch_shapes(wpb, rows):
     try:
         ctx.set_tuning(wpb, rows)
         assert ctx.get_tuning() == (wpb, rows)
-        got = P.migrate_volume(data, vel, prm, ctx=ctx)
-        assert rel_l2(got, ref) <= TOL
+        # both forward kernel forms (the staged one exists for 4 and 8 crosslines
+        # per workgroup; other widths gather)
+        for staged in (False, True):
+            ctx.set_staging(staged, 2)
+            got = P.migrate_volume(data, vel, prm, ctx=ctx)
+            assert rel_l2(got, ref) <= TOL, staged
+        ctx.set_staging(None, 2)
         # the per-inline drop-in under the same shape
         vmax = O.estimate_max_velocity(vel)
         tan2, ap = O.aperture(45.0, vmax, n[2], 0.002, 25.0)

@@ -1,0 +1,146 @@
+"""The LDS-staged forward kernel against the gather kernel and the reference.
+
+Both kernel forms evaluate the same terms (same mask, travel time, sample
+index, interpolation and weight per pair) and differ only in FP32 summation
+order, so they agree to rounding (rel-L2 <= 2e-6 here); each matches the
+reference kernel to the parity bar (1e-5).  The cases reach every path of the
+staged kernel: sticky trailing crosslines (the workgroup runs the gather
+path), coarse crossline bins with a slow velocity (time windows wider than a
+staged slot: per-row fallback), fine crossline bins (more column offsets than
+the column table holds: per-row fallback), and both storage forms.
+"""
+import numpy as np
+import pytest
+
+from _cases import reference_volume, rel_l2
+
+import poststackmig3d_amd as P
+from poststackmig3d_amd import synth
+from poststackmig3d_amd._lib import Context, PsmError
+
+pytestmark = pytest.mark.gpu
+
+TOL_REF = 1e-5
+TOL_FORMS = 2e-6
+
+
+def _case(name):
+    if name == "padded_lateral":
+        n = (29, 31, 140)
+        data, _ = synth.production(*n, n_diffractors=4, seed=8)
+        vel = synth.production_velocity(*n)
+        geo = dict(inline_step=25.0, crossline_step=25.0, inline_padding=3, crossline_padding=4,
+                   angle_aperture=45.0)
+    elif name == "point_unpadded":
+        n = (41, 45, 160)
+        data, vel = synth.point_diffractor(*n, noise=0.1, seed=4)
+        geo = dict(inline_step=25.0, crossline_step=25.0, angle_aperture=30.0)
+    elif name == "coarse_slow":
+        # 200 m crossline bins at 1000 m/s: one crossline step moves the travel
+        # time by up to 2 sin(70) 200 / 1000 s = 188 samples, so the windows of
+        # columns read by several waves exceed a slot
+        n = (15, 17, 600)
+        data, _ = synth.point_diffractor(*n, dy=200.0, v=1000.0, noise=0.2, seed=5)
+        vel = np.full(n, 1000.0, np.float32)
+        geo = dict(inline_step=25.0, crossline_step=200.0, crossline_padding=2,
+                   angle_aperture=70.0)
+    elif name == "fine_bins":
+        # 0.5 m crossline bins: the aperture spans more crossline offsets than
+        # the column table holds
+        n = (3, 1100, 300)
+        data, _ = synth.point_diffractor(*n, dy=0.5, noise=0.1, seed=6)
+        vel = np.full(n, 2000.0, np.float32)
+        geo = dict(inline_step=25.0, crossline_step=0.5, angle_aperture=60.0)
+    else:
+        raise KeyError(name)
+    prm = P.SurveyParams.for_cube(n, **geo)
+    ref_args = (0.002, geo["inline_step"], geo["crossline_step"], geo.get("inline_padding", 0),
+                geo.get("crossline_padding", 0), geo["angle_aperture"], True)
+    return data, vel, prm, ref_args
+
+
+CASE_NAMES = ["padded_lateral", "point_unpadded", "coarse_slow", "fine_bins"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _migrate(ctx, data, vel, prm, staging, storage="f32", wpb=4, steps=4, rows=-1):
+    ctx.set_storage(storage)
+    ctx.set_tuning(wpb, rows)
+    ctx.set_staging(staging, steps)
+    try:
+        return P.migrate_volume(data, vel, prm, ctx=ctx)
+    finally:
+        ctx.set_storage("f32")
+        ctx.set_tuning(4, -1)
+        ctx.set_staging(None, 2)
+
+
+@pytest.mark.parametrize("name", CASE_NAMES)
+def test_staged_matches_reference_and_gather(ctx, name):
+    data, vel, prm, ref_args = _case(name)
+    print(f"staging case {name} {data.shape}", flush=True)
+    ref = reference_volume(data, vel, *ref_args)
+    gather = _migrate(ctx, data, vel, prm, staging=False)
+    assert rel_l2(gather, ref) <= TOL_REF
+    for wpb in (4, 8):
+        for steps in (2, 4):
+            got = _migrate(ctx, data, vel, prm, staging=True, wpb=wpb, steps=steps)
+            assert np.isfinite(got).all()
+            e_ref, e_g = rel_l2(got, ref), rel_l2(got, gather)
+            print(f"  wpb {wpb} steps {steps}: vs reference {e_ref:.2e}, vs gather {e_g:.2e}",
+                  flush=True)
+            assert e_ref <= TOL_REF, (wpb, steps, e_ref)
+            assert e_g <= TOL_FORMS, (wpb, steps, e_g)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("name", ["padded_lateral", "coarse_slow"])
+def test_staged_half_storage_matches_gather(ctx, name, storage):
+    data, vel, prm, _ = _case(name)
+    gather = _migrate(ctx, data, vel, prm, staging=False, storage=storage)
+    for wpb, steps in ((4, 4), (8, 2)):
+        got = _migrate(ctx, data, vel, prm, staging=True, storage=storage, wpb=wpb, steps=steps)
+        e = rel_l2(got, gather)
+        print(f"  {name} {storage} wpb {wpb} steps {steps}: vs gather {e:.2e}", flush=True)
+        assert e <= TOL_FORMS
+
+
+def test_staged_row_passes(ctx):
+    """Aperture-row passes (later launches accumulate) with the staged form."""
+    data, vel, prm, ref_args = _case("padded_lateral")
+    ref = reference_volume(data, vel, *ref_args)
+    for rows in (0, 1, 3):
+        got = _migrate(ctx, data, vel, prm, staging=True, rows=rows)
+        assert rel_l2(got, ref) <= TOL_REF, rows
+
+
+def test_staging_switch_roundtrip(ctx):
+    assert ctx.get_staging() == (None, 2)  # default: chosen by trace storage
+    ctx.set_staging(False, 2)
+    assert ctx.get_staging() == (False, 2)
+    ctx.set_staging(True, 4)
+    assert ctx.get_staging() == (True, 4)
+    with pytest.raises(PsmError):
+        ctx.set_staging(True, 3)
+    ctx.set_staging(None, 2)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_automatic_form_matches_forced_forms(ctx, storage):
+    """The default (automatic) form equals the form it selects: staged with 8
+    crosslines per workgroup and 2 offsets per group for FP32 traces, gather
+    for half-precision traces."""
+    data, vel, prm, _ = _case("padded_lateral")
+    auto = _migrate(ctx, data, vel, prm, staging=None, storage=storage)
+    if storage == "f32":
+        forced = _migrate(ctx, data, vel, prm, staging=True, storage=storage, wpb=8, steps=2,
+                          rows=0)
+    else:
+        forced = _migrate(ctx, data, vel, prm, staging=False, storage=storage)
+    assert np.array_equal(auto, forced)
