@@ -654,16 +654,24 @@ template <int ST, int W, int G>
 struct LdsPlan {
   static constexpr int kB = Store<ST>::kBytes;
   static constexpr int kSlot = kWinPairs * kB;           // bytes of one window
-  static constexpr int kRing = 2 * G + W - 1;            // slots per (ring row, side)
+  static constexpr int kCols = kMaxCols + G + W;         // table entries (reads past nO stay inside)
+  static constexpr int kTables = 3 * kCols * 4 + 5 * W * 4 + W * 4;
+  // Double ring (2G + W - 1 slots per ring row and side): the next group's
+  // windows are stored while the current group is read, one barrier per
+  // group.  Single ring (G + W - 1 slots): they are stored after a barrier
+  // that ends the current group's reads, two barriers per group; used when
+  // the double ring would leave fewer than three workgroups per CU.
+  static constexpr bool kSingle = 4 * (2 * G + W - 1) * kSlot + kTables > 53 * 1024;
+  static constexpr int kRing = kSingle ? G + W - 1 : 2 * G + W - 1;
   static constexpr int kSide = kRing * kSlot;            // side 0 -> side 1
   static constexpr int kRow = 2 * kSide;                 // ring row 0 -> ring row 1
-  static constexpr int kMeta = 2 * kRow;                 // int meta[2][kMaxCols]
-  static constexpr int kR2 = kMeta + 2 * kMaxCols * 4;   // float r2tab[kMaxCols]
-  static constexpr int kPar = kR2 + kMaxCols * 4;        // float par[5][W]
+  static constexpr int kMeta = 2 * kRow;                 // int meta[2][kCols]
+  static constexpr int kR2 = kMeta + 2 * kCols * 4;      // float r2tab[kCols]
+  static constexpr int kPar = kR2 + kCols * 4;           // float par[5][W]
   static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
   static constexpr int kBytes = kFlag + W * 4;
   static_assert(kRow <= 65535, "ring row offset must fit a ds_read offset");
-  static_assert(kSlot % (16 * 1) == 0 && kSlot <= 16 * kWave, "one load per window");
+  static_assert(kSlot <= 16 * kWave, "one load per window");
   __device__ static int slot(int side, int o) { return side * kSide + (o % kRing) * kSlot; }
 };
 
@@ -756,12 +764,12 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
     const char* const zero_row = base + (size_t)a.n_rows * row_bytes;
     const int row_tr = a.n_xd + 1;
     const int kbase = a.cur_xl_start + j0;  // buffer crossline of side-0 column offset 0
-    // A lane without a valid sample reads at the lower corner of its wave's
-    // bounds, inside every window it touches; its terms are masked (R2 = -1).
+    // A lane without a valid sample reads at the lower corner of the
+    // workgroup's bounds, inside every window; its terms are masked (R2 = -1).
     LaneParams p = ls.p;
     if (!valid) {
-      p.t0sq = Tlo2;
-      p.a = Alo;
+      p.t0sq = gTlo2;
+      p.a = gAlo;
     }
 
     int dmax = (int)(__builtin_sqrtf(R2wg) / a.dx) + 2;
@@ -791,7 +799,6 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
       }
       const int kmax_wg = uniform_i(wave_max_i(km));
       if (kmax_wg < 0) continue;
-      const int kmax_w = __builtin_amdgcn_readlane(km, w);
       const int nO = kmax_wg + W;  // column offsets per side
       const int r0 = hp ? rp : rm;  // ring row 0; ring row 1 = rm when both rows exist
       const bool two = hp && hm;
@@ -825,7 +832,7 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
                 const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
                 ovf = ovf || (hi - lo >= kWinPairs);
               }
-              meta[side * kMaxCols + o] = PL::slot(side, o) - lo * kB;
+              meta[side * PL::kCols + o] = PL::slot(side, o) - lo * kB;
             }
           }
         }
@@ -854,8 +861,11 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
             const_cast<char*>(base + (size_t)rm * row_bytes), 0, (int)row_bytes, kRsrcWord3);
         // A job's source: column o, side, ring row rr of the job index j
         // within the columns [oa, oa + nc).
+        // (one return statement: a lambda returning this struct on two paths
+        // made the compiler copy the kernel arguments to scratch)
+        const int jsh = two ? 2 : 1;  // log2 of the jobs per column
         auto job_at = [&](int oa, int nc, int j) {
-          return StageJob{oa + j / (2 * nr), (j / nr) % 2, j % nr, j < nc * 2 * nr};
+          return StageJob{oa + (j >> jsh), (j >> (jsh - 1)) & 1, j & (jsh - 1), j < (nc << jsh)};
         };
         // buffer offsets of a job whose table entry is m (meta of its column)
         auto job_src = [&](const StageJob& jb, int m, int& voff, int& soff) {
@@ -867,7 +877,7 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
           voff = zero ? 0x7ffffff0 : lo * kB + lb;
         };
         auto job_dst = [&](const StageJob& jb) { return PL::slot(jb.side, jb.o) + jb.rr * PL::kRow; };
-        auto tab = [&](int side, int o) { return meta[side * kMaxCols + min(o, kMaxCols - 1)]; };
+        auto tab = [&](int side, int o) { return meta[side * PL::kCols + o]; };
         {  // the first G + W - 1 columns, loaded straight into LDS (no
            // registers held); complete before the barrier
           constexpr int kFill = (4 * (G + W - 1) + W - 1) / W;
@@ -904,7 +914,7 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         auto read_tab = [&](int dk0, Tab& t) {
 #pragma unroll
           for (int u = 0; u < G; ++u) {
-            const int dk = min(dk0 + u, kMaxCols - W);
+            const int dk = dk0 + u;
             t.r2[u] = r2tab[dk];
             t.bR[u] = tab(0, w + dk);
             t.bL[u] = tab(1, (W - 1 - w) + dk);
@@ -917,8 +927,15 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
             t.mj[q] = jb.ok ? tab(jb.side, jb.o) : 0;
           }
         };
-        auto group = [&](auto nr_c, int dk0, int dk_end, const Tab& t) {
+        // Steps dk0 + u, u < n (FULL: all G).  Every wave sweeps the
+        // workgroup's offsets: each column's window covers all waves' reads up
+        // to kmax_wg, and a step past a wave's own last offset adds only masked
+        // terms or exact zeros (pairs past the trace end).  FIRST: the group
+        // holds dk = 0, whose -dk column is the +dk column and is left out.
+        auto compute = [&](auto nr_c, auto full_c, auto first_c, int dk0, int n, const Tab& t) {
           constexpr int NR = decltype(nr_c)::value;
+          constexpr bool FULL = decltype(full_c)::value;
+          constexpr bool FIRST = decltype(first_c)::value;
           struct Stp {
             float f, wt;
             bool m;
@@ -928,8 +945,9 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
           Stp st[G];
 #pragma unroll
           for (int u = 0; u < G; ++u) {
-            const int dk = dk0 + u;
-            if (dk < dk_end) {
+            if (FULL || u < n) {
+              const int dk = dk0 + u;
+              const bool one = FIRST && u == 0;  // dk = 0: one column
               const float r2 = t.r2[u];
               const float tau = sqrt_up(__builtin_fmaf(p.a, r2, p.t0sq));
               const float q = tau * a.inv_dt;
@@ -937,15 +955,15 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
               const char* xr = smem + t.bR[u] + ib;
               const char* xl = smem + t.bL[u] + ib;
               st[u].x[0] = *reinterpret_cast<const Raw*>(xr);
-              st[u].x[1] = dk > 0 ? *reinterpret_cast<const Raw*>(xl) : Raw{};
+              st[u].x[1] = one ? Raw{} : *reinterpret_cast<const Raw*>(xl);
               if constexpr (NR == 2) {
                 st[u].x[2] = *reinterpret_cast<const Raw*>(xr + PL::kRow);
-                st[u].x[3] = dk > 0 ? *reinterpret_cast<const Raw*>(xl + PL::kRow) : Raw{};
+                st[u].x[3] = one ? Raw{} : *reinterpret_cast<const Raw*>(xl + PL::kRow);
               }
               if constexpr (S::kScaled) {
                 int cR = kj + dk - xd_lo, cL = kj - dk - xd_lo;
                 cR = (unsigned)cR < (unsigned)a.n_xd ? cR : a.n_xd;
-                cL = (dk > 0 && (unsigned)cL < (unsigned)a.n_xd) ? cL : a.n_xd;
+                cL = (!one && (unsigned)cL < (unsigned)a.n_xd) ? cL : a.n_xd;
                 st[u].sc[0] = sc0[cR];
                 st[u].sc[1] = sc0[cL];
                 if constexpr (NR == 2) {
@@ -958,65 +976,83 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
               st[u].m = r2 <= p.R2;
             }
           }
-          // slot order (row, column) = (+db,+dk), (+db,-dk), (-db,+dk),
-          // (-db,-dk); the -dk column of dk = 0 holds zeros
+          // slot order (row, column) = (+db,+dk), (+db,-dk), (-db,+dk), (-db,-dk)
 #pragma unroll
           for (int u = 0; u < G; ++u) {
-            if (dk0 + u < dk_end) {
+            if (FULL || u < n) {
               const Stp& q = st[u];
-              float l[2 * NR];
-#pragma unroll
-              for (int k = 0; k < 2 * NR; ++k) l[k] = S::lerp(q.x[k], q.f);
+              const bool one = FIRST && u == 0;
               float sum;
               if constexpr (S::kScaled) {
-                sum = l[0] * q.sc[0];
-#pragma unroll
-                for (int k = 1; k < 2 * NR; ++k) sum = __builtin_fmaf(l[k], q.sc[k], sum);
+                sum = S::lerp(q.x[0], q.f) * q.sc[0];
+                if (!one) sum = __builtin_fmaf(S::lerp(q.x[1], q.f), q.sc[1], sum);
+                if constexpr (NR == 2) {
+                  sum = __builtin_fmaf(S::lerp(q.x[2], q.f), q.sc[2], sum);
+                  if (!one) sum = __builtin_fmaf(S::lerp(q.x[3], q.f), q.sc[3], sum);
+                }
+              } else if (one) {
+                sum = S::lerp(q.x[0], q.f);
+                if constexpr (NR == 2) sum = sum + S::lerp(q.x[2], q.f);
               } else if constexpr (NR == 2) {
-                sum = (l[0] + l[1]) + (l[2] + l[3]);
+                sum = (S::lerp(q.x[0], q.f) + S::lerp(q.x[1], q.f)) +
+                      (S::lerp(q.x[2], q.f) + S::lerp(q.x[3], q.f));
               } else {
-                sum = l[0] + l[1];
+                sum = S::lerp(q.x[0], q.f) + S::lerp(q.x[1], q.f);
               }
               sum = q.m ? sum : 0.0f;
               acc = __builtin_fmaf(sum, q.wt, acc);
             }
           }
         };
-        Tab cur;
-        read_tab(0, cur);
-        __syncthreads();  // the first windows are visible
-        for (int dk0 = 0; dk0 <= kmax_wg; dk0 += G) {
-          // the next group's G new columns, in flight while this group runs;
-          // their slots are not read by this group (ring of 2G + W - 1 slots)
+        // one group: the next group's G new columns are loaded while this
+        // group runs and stored into slots this group does not read (double
+        // ring) or, after a barrier, into the slots of the G columns no later
+        // group reads (single ring).  Every wave issues the same
+        // loads whether or not its jobs exist (absent jobs read out of range),
+        // so no loaded value is merged across a branch and nothing waits on
+        // the loads before the stores.
+        auto run_group = [&](auto nr_c, auto full_c, auto first_c, int dk0, Tab& t) {
           const int oa = dk0 + G + W - 1;
           const int nc = max(0, min(G, nO - oa));
-          // every wave issues the same loads whether or not its jobs exist
-          // (absent jobs read out of range), so no loaded value is merged
-          // across a branch and nothing waits on the loads before the stores
           V4 v[kJobs];
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
             const StageJob jb = job_at(oa, nc, w + q * W);
             int voff, soff;
-            job_src(jb, cur.mj[q], voff, soff);
+            job_src(jb, t.mj[q], voff, soff);
             v[q] = __builtin_amdgcn_raw_buffer_load_b128(jb.rr == 0 ? rs0 : rs1,
                                                          jb.ok ? voff : 0x7ffffff0, soff, 0);
           }
-          const int dk_end = min(dk0 + G, kmax_w + 1);
-          if (dk0 < dk_end) {
-            if (two)
-              group(std::integral_constant<int, 2>{}, dk0, dk_end, cur);
-            else
-              group(std::integral_constant<int, 1>{}, dk0, dk_end, cur);
-          }
-          read_tab(dk0 + G, cur);
+          compute(nr_c, full_c, first_c, dk0, min(G, kmax_wg + 1 - dk0), t);
+          read_tab(dk0 + G, t);
+          // the stores wait on the loads: keep them after this group's arithmetic
+          __builtin_amdgcn_sched_barrier(0);
+          if constexpr (PL::kSingle) __syncthreads();  // this group's reads are done
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
             const StageJob jb = job_at(oa, nc, w + q * W);
             if (jb.ok && lb < PL::kSlot) *reinterpret_cast<V4*>(smem + job_dst(jb) + lb) = v[q];
           }
           __syncthreads();  // the next group's windows are visible
-        }
+        };
+        auto run_row = [&](auto nr_c) {
+          using T = std::integral_constant<bool, true>;
+          using F = std::integral_constant<bool, false>;
+          Tab t;
+          read_tab(0, t);
+          __syncthreads();  // the first windows are visible
+          if (kmax_wg + 1 >= G)
+            run_group(nr_c, T{}, T{}, 0, t);
+          else
+            run_group(nr_c, F{}, T{}, 0, t);
+          int dk0 = G;
+          for (; dk0 + G <= kmax_wg + 1; dk0 += G) run_group(nr_c, T{}, F{}, dk0, t);
+          if (dk0 <= kmax_wg) run_group(nr_c, F{}, F{}, dk0, t);
+        };
+        if (two)
+          run_row(std::integral_constant<int, 2>{});
+        else
+          run_row(std::integral_constant<int, 1>{});
         continue;
       }
       // gather path for this row
