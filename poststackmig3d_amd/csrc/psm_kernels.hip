@@ -259,7 +259,7 @@ __device__ __forceinline__ void sweep_row(const MigArgs& a, int kj, const LanePa
           col[c] = ok ? col[c] : a.n_xd;
           off[c] = col[c] * tb;
         }
-        const float tau = sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq));
+        const float tau = sqrt_up(__builtin_fmaf(p.a, r2, p.t0sq));
         const float q = tau * a.inv_dt;
         const int voff = min((int)q, nt1) * S::kBytes;
 #pragma unroll
