@@ -661,11 +661,12 @@ struct LdsPlan {
   // group.  Single ring (G + W - 1 slots): they are stored after a barrier
   // that ends the current group's reads, two barriers per group; used when
   // the double ring would leave fewer than three workgroups per CU.
-  static constexpr bool kSingle = 4 * (2 * G + W - 1) * kSlot + kTables > 53 * 1024;
+  static constexpr bool kSingle = (4 * (2 * G + W - 1) + 1) * kSlot + kTables > 53 * 1024;
   static constexpr int kRing = kSingle ? G + W - 1 : 2 * G + W - 1;
   static constexpr int kSide = kRing * kSlot;            // side 0 -> side 1
   static constexpr int kRow = 2 * kSide;                 // ring row 0 -> ring row 1
-  static constexpr int kMeta = 2 * kRow;                 // int meta[2][kCols]
+  static constexpr int kSpare = 2 * kRow;                // one slot that absent jobs write
+  static constexpr int kMeta = kSpare + kSlot;           // int meta[2][kCols]
   static constexpr int kR2 = kMeta + 2 * kCols * 4;      // float r2tab[kCols]
   static constexpr int kPar = kR2 + kCols * 4;           // float par[5][W]
   static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
@@ -1025,13 +1026,18 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
           }
           compute(nr_c, full_c, first_c, dk0, min(G, kmax_wg + 1 - dk0), t);
           read_tab(dk0 + G, t);
-          // the stores wait on the loads: keep them after this group's arithmetic
+          // The stores wait on the loads, so the group's sum must be complete
+          // before them: the empty asm takes the accumulator as an input, and
+          // no store is conditional (absent jobs write the spare slot), so no
+          // summation is sunk past a store.
+          asm volatile("" ::"v"(acc));
           __builtin_amdgcn_sched_barrier(0);
           if constexpr (PL::kSingle) __syncthreads();  // this group's reads are done
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
             const StageJob jb = job_at(oa, nc, w + q * W);
-            if (jb.ok && lb < PL::kSlot) *reinterpret_cast<V4*>(smem + job_dst(jb) + lb) = v[q];
+            const int dst = jb.ok ? job_dst(jb) : PL::kSpare;
+            if (lb < PL::kSlot) *reinterpret_cast<V4*>(smem + dst + lb) = v[q];
           }
           __syncthreads();  // the next group's windows are visible
         };
