@@ -856,10 +856,15 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         using V4 = unsigned __attribute__((ext_vector_type(4)));
         const int nr = two ? 2 : 1;
         const int lb = lane * 16;
+        const int lbs = lb < PL::kSlot ? lb : 0x7ff00000;  // lanes past the slot load nothing
         const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<char*>(base + (size_t)r0 * row_bytes), 0, (int)row_bytes, kRsrcWord3);
         const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<char*>(base + (size_t)rm * row_bytes), 0, (int)row_bytes, kRsrcWord3);
+        // no records: every load through it reads zeros (absent jobs, columns
+        // outside the data)
+        const __amdgpu_buffer_rsrc_t rsz =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), 0, 0, kRsrcWord3);
         // A job's source: column o, side, ring row rr of the job index j
         // within the columns [oa, oa + nc).
         // (one return statement: a lambda returning this struct on two paths
@@ -911,6 +916,20 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         // addresses of each step, and the table entries of the staging jobs
         // for the group after it.
         constexpr int kJobs = (4 * G + W - 1) / W;  // staging jobs per wave per group, at most
+        // This wave's jobs in every full group of the row: job q is column
+        // oa + jc[q] of side js[q], ring row jr[q], if jh[q] (oa = the
+        // group's first new column); the same in every group, so it is set
+        // once per row in scalar registers.
+        int jc[kJobs], js[kJobs], jr[kJobs];
+        bool jh[kJobs];
+#pragma unroll
+        for (int q = 0; q < kJobs; ++q) {
+          const StageJob jb = job_at(0, G, w + q * W);
+          jc[q] = uniform_i(jb.o);
+          js[q] = uniform_i(jb.side);
+          jr[q] = uniform_i(jb.rr);
+          jh[q] = jb.ok;
+        }
         using Tab = StageTab<G, kJobs>;
         auto read_tab = [&](int dk0, Tab& t) {
 #pragma unroll
@@ -920,13 +939,11 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
             t.bR[u] = tab(0, w + dk);
             t.bL[u] = tab(1, (W - 1 - w) + dk);
           }
+          // (read whether or not the job exists: the index stays inside the
+          // table, and an absent job loads through the zero resource)
           const int oa = dk0 + G + W - 1;
-          const int nc = max(0, min(G, nO - oa));
 #pragma unroll
-          for (int q = 0; q < kJobs; ++q) {
-            const StageJob jb = job_at(oa, nc, w + q * W);
-            t.mj[q] = jb.ok ? tab(jb.side, jb.o) : 0;
-          }
+          for (int q = 0; q < kJobs; ++q) t.mj[q] = tab(js[q], oa + jc[q]);
         };
         // Steps dk0 + u, u < n (FULL: all G).  Every wave sweeps the
         // workgroup's offsets: each column's window covers all waves' reads up
@@ -1014,15 +1031,23 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         // the loads before the stores.
         auto run_group = [&](auto nr_c, auto full_c, auto first_c, int dk0, Tab& t) {
           const int oa = dk0 + G + W - 1;
-          const int nc = max(0, min(G, nO - oa));
           V4 v[kJobs];
+          int dst[kJobs];
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
-            const StageJob jb = job_at(oa, nc, w + q * W);
-            int voff, soff;
-            job_src(jb, t.mj[q], voff, soff);
-            v[q] = __builtin_amdgcn_raw_buffer_load_b128(jb.rr == 0 ? rs0 : rs1,
-                                                         jb.ok ? voff : 0x7ffffff0, soff, 0);
+            // a job loads pairs lo .. lo + kWinPairs - 1 of its column (lo * kB
+            // = slot - table entry); pairs past the trace end land in slot
+            // entries no read reaches (reads stop at hi <= n_t - 1) and come
+            // from the next trace of the row, which holds at least kSlot bytes
+            const int o = oa + jc[q];
+            const int k = js[q] == 0 ? kbase + o : kbase + (W - 1) - o;
+            const bool ok = jh[q] && o < nO;
+            const bool data = ok && k >= xd_lo && k < xd_hi;
+            const int sl = PL::slot(js[q], o);
+            dst[q] = ok ? sl + jr[q] * PL::kRow : PL::kSpare;
+            v[q] = __builtin_amdgcn_raw_buffer_load_b128(data ? (jr[q] == 0 ? rs0 : rs1) : rsz,
+                                                         sl + lbs - t.mj[q],
+                                                         data ? (k - xd_lo) * tb : 0, 0);
           }
           compute(nr_c, full_c, first_c, dk0, min(G, kmax_wg + 1 - dk0), t);
           read_tab(dk0 + G, t);
@@ -1034,11 +1059,8 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
           __builtin_amdgcn_sched_barrier(0);
           if constexpr (PL::kSingle) __syncthreads();  // this group's reads are done
 #pragma unroll
-          for (int q = 0; q < kJobs; ++q) {
-            const StageJob jb = job_at(oa, nc, w + q * W);
-            const int dst = jb.ok ? job_dst(jb) : PL::kSpare;
-            if (lb < PL::kSlot) *reinterpret_cast<V4*>(smem + dst + lb) = v[q];
-          }
+          for (int q = 0; q < kJobs; ++q)
+            if (lb < PL::kSlot) *reinterpret_cast<V4*>(smem + dst[q] + lb) = v[q];
           __syncthreads();  // the next group's windows are visible
         };
         auto run_row = [&](auto nr_c) {
@@ -1338,7 +1360,10 @@ hipError_t launch_migrate(const MigArgs& a0, int mode, hipStream_t stream, const
   // LDS-staged forward kernel: contiguous rows, 4 or 8 output crosslines per
   // workgroup, rows addressable by a 32-bit buffer size
   const size_t row_bytes = (size_t)(a0.n_xd + 1) * a0.n_t * (a0.storage == PSM_STORE_F32 ? 8 : 4);
-  const bool lds_ok = mode == kForward && a0.row_pos == nullptr && row_bytes <= kLdsMaxRowBytes;
+  // (a staging load reads up to kWinPairs pairs from its window start, which
+  // can pass the trace end by that much: traces must be at least that long)
+  const bool lds_ok = mode == kForward && a0.row_pos == nullptr && row_bytes <= kLdsMaxRowBytes &&
+                      a0.n_t >= kWinPairs;
   const bool auto_lds = tune.staging < 0 && a0.storage == PSM_STORE_F32 && lds_ok;
   const int wpb = auto_lds ? kAutoStageWaves : tune.waves_per_block;
   const int steps = auto_lds ? kAutoStageSteps : tune.stage_steps;
