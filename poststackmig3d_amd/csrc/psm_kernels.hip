@@ -939,9 +939,11 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
             t.bR[u] = tab(0, w + dk);
             t.bL[u] = tab(1, (W - 1 - w) + dk);
           }
-          // (read whether or not the job exists: the index stays inside the
-          // table, and an absent job loads through the zero resource)
-          const int oa = dk0 + G + W - 1;
+          // job entries of the loads issued in group dk0 (the columns group
+          // dk0 + 2G adds); read whether or not the job exists: the index
+          // stays inside the table, and an absent job loads through the zero
+          // resource
+          const int oa = dk0 + 2 * G + W - 1;
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) t.mj[q] = tab(js[q], oa + jc[q]);
         };
@@ -1029,10 +1031,13 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         // loads whether or not its jobs exist (absent jobs read out of range),
         // so no loaded value is merged across a branch and nothing waits on
         // the loads before the stores.
-        auto run_group = [&](auto nr_c, auto full_c, auto first_c, int dk0, Tab& t) {
-          const int oa = dk0 + G + W - 1;
-          V4 v[kJobs];
-          int dst[kJobs];
+        // The new columns of group dk0 + G (oa = dk0 + 2G + W - 1... for the
+        // loads issued in group dk0) are loaded two groups ahead of their
+        // reads: issued at the start of group dk0 - G, held in registers
+        // through that group and group dk0, stored at the end of group dk0.
+        using Jobs = V4[kJobs];
+        using Dsts = int[kJobs];
+        auto issue = [&](int oa, const int* m, Jobs& v, Dsts& dst) {
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
             // a job loads pairs lo .. lo + kWinPairs - 1 of its column (lo * kB
@@ -1046,9 +1051,16 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
             const int sl = PL::slot(js[q], o);
             dst[q] = ok ? sl + jr[q] * PL::kRow : PL::kSpare;
             v[q] = __builtin_amdgcn_raw_buffer_load_b128(data ? (jr[q] == 0 ? rs0 : rs1) : rsz,
-                                                         sl + lbs - t.mj[q],
+                                                         sl + lbs - m[q],
                                                          data ? (k - xd_lo) * tb : 0, 0);
           }
+        };
+        // one group: issue the loads for group dk0 + 2G (into vi), sum the
+        // group, read the next group's tables, store the loads issued one
+        // group earlier (vs) and make them visible.
+        auto run_group = [&](auto nr_c, auto full_c, auto first_c, int dk0, Tab& t, Jobs& vs,
+                             Dsts& ds, Jobs& vi, Dsts& di) {
+          issue(dk0 + 2 * G + W - 1, t.mj, vi, di);
           compute(nr_c, full_c, first_c, dk0, min(G, kmax_wg + 1 - dk0), t);
           read_tab(dk0 + G, t);
           // The stores wait on the loads, so the group's sum must be complete
@@ -1060,7 +1072,7 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
           if constexpr (PL::kSingle) __syncthreads();  // this group's reads are done
 #pragma unroll
           for (int q = 0; q < kJobs; ++q)
-            if (lb < PL::kSlot) *reinterpret_cast<V4*>(smem + dst[q] + lb) = v[q];
+            if (lb < PL::kSlot) *reinterpret_cast<V4*>(smem + ds[q] + lb) = vs[q];
           __syncthreads();  // the next group's windows are visible
         };
         auto run_row = [&](auto nr_c) {
@@ -1068,14 +1080,31 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
           using F = std::integral_constant<bool, false>;
           Tab t;
           read_tab(0, t);
+          Jobs va, vb;
+          Dsts da, db;
+          {  // the loads stored at the end of group 0
+            int m0[kJobs];
+#pragma unroll
+            for (int q = 0; q < kJobs; ++q) m0[q] = tab(js[q], G + W - 1 + jc[q]);
+            issue(G + W - 1, m0, va, da);
+          }
           __syncthreads();  // the first windows are visible
           if (kmax_wg + 1 >= G)
-            run_group(nr_c, T{}, T{}, 0, t);
+            run_group(nr_c, T{}, T{}, 0, t, va, da, vb, db);
           else
-            run_group(nr_c, F{}, T{}, 0, t);
+            run_group(nr_c, F{}, T{}, 0, t, va, da, vb, db);
+          // groups alternate the two register sets (unrolled by two)
           int dk0 = G;
-          for (; dk0 + G <= kmax_wg + 1; dk0 += G) run_group(nr_c, T{}, F{}, dk0, t);
-          if (dk0 <= kmax_wg) run_group(nr_c, F{}, F{}, dk0, t);
+          for (; dk0 + 2 * G <= kmax_wg + 1; dk0 += 2 * G) {
+            run_group(nr_c, T{}, F{}, dk0, t, vb, db, va, da);
+            run_group(nr_c, T{}, F{}, dk0 + G, t, va, da, vb, db);
+          }
+          if (dk0 + G <= kmax_wg + 1) {
+            run_group(nr_c, T{}, F{}, dk0, t, vb, db, va, da);
+            if (dk0 + G <= kmax_wg) run_group(nr_c, F{}, F{}, dk0 + G, t, va, da, vb, db);
+          } else if (dk0 <= kmax_wg) {
+            run_group(nr_c, F{}, F{}, dk0, t, vb, db, va, da);
+          }
         };
         if (two)
           run_row(std::integral_constant<int, 2>{});
