@@ -654,7 +654,7 @@ template <int ST, int W, int G>
 struct LdsPlan {
   static constexpr int kB = Store<ST>::kBytes;
   static constexpr int kSlot = kWinPairs * kB;           // bytes of one window
-  static constexpr int kCols = kMaxCols + G + W;         // table entries (reads past nO stay inside)
+  static constexpr int kCols = kMaxCols + 4 * G + W;     // table entries (reads past nO stay inside)
   static constexpr int kTables = 3 * kCols * 4 + 5 * W * 4 + W * 4;
   // Double ring (2G + W - 1 slots per ring row and side): the next group's
   // windows are stored while the current group is read, one barrier per
@@ -861,10 +861,6 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
             const_cast<char*>(base + (size_t)r0 * row_bytes), 0, (int)row_bytes, kRsrcWord3);
         const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<char*>(base + (size_t)rm * row_bytes), 0, (int)row_bytes, kRsrcWord3);
-        // no records: every load through it reads zeros (absent jobs, columns
-        // outside the data)
-        const __amdgpu_buffer_rsrc_t rsz =
-            __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), 0, 0, kRsrcWord3);
         // A job's source: column o, side, ring row rr of the job index j
         // within the columns [oa, oa + nc).
         // (one return statement: a lambda returning this struct on two paths
@@ -917,18 +913,32 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         // for the group after it.
         constexpr int kJobs = (4 * G + W - 1) / W;  // staging jobs per wave per group, at most
         // This wave's jobs in every full group of the row: job q is column
-        // oa + jc[q] of side js[q], ring row jr[q], if jh[q] (oa = the
-        // group's first new column); the same in every group, so it is set
-        // once per row in scalar registers.
-        int jc[kJobs], js[kJobs], jr[kJobs];
-        bool jh[kJobs];
+        // oa + jc of side js, ring row jr, if the wave has that job at all
+        // (oa = the group's first new column), so the jobs' state advances
+        // by fixed steps from group to group and is kept in scalar
+        // registers: jo = column offset of the next load, jsl = its ring slot
+        // (+ ring row offset), jkd = its data crossline (buffer crossline -
+        // xd_lo), jso = jkd * tb.  A column outside the data loads the row's
+        // zero trace (its last); a job past the row's columns (or a wave
+        // without the job) loads a column like any other and stores into the
+        // spare slot.
+        int js[kJobs], jo[kJobs], jsl[kJobs], jkd[kJobs], jso[kJobs];
+        int jlim[kJobs], jslend[kJobs], jdkd[kJobs], jlbs[kJobs];
+        __amdgpu_buffer_rsrc_t jrs[kJobs];
 #pragma unroll
         for (int q = 0; q < kJobs; ++q) {
           const StageJob jb = job_at(0, G, w + q * W);
-          jc[q] = uniform_i(jb.o);
+          const int jc = uniform_i(jb.o), jr = uniform_i(jb.rr);
           js[q] = uniform_i(jb.side);
-          jr[q] = uniform_i(jb.rr);
-          jh[q] = jb.ok;
+          jo[q] = G + W - 1 + jc;
+          jlim[q] = jb.ok ? nO : -1;
+          jsl[q] = PL::slot(js[q], jo[q]) + jr * PL::kRow;
+          jslend[q] = js[q] * PL::kSide + jr * PL::kRow + PL::kRing * PL::kSlot;
+          jkd[q] = (js[q] == 0 ? kbase + jo[q] : kbase + (W - 1) - jo[q]) - xd_lo;
+          jdkd[q] = js[q] == 0 ? G : -G;
+          jso[q] = jkd[q] * tb;
+          jrs[q] = jr == 0 ? rs0 : rs1;
+          jlbs[q] = lbs - jr * PL::kRow;  // load offsets are relative to the side's slot
         }
         using Tab = StageTab<G, kJobs>;
         auto read_tab = [&](int dk0, Tab& t) {
@@ -939,13 +949,10 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
             t.bR[u] = tab(0, w + dk);
             t.bL[u] = tab(1, (W - 1 - w) + dk);
           }
-          // job entries of the loads issued in group dk0 (the columns group
-          // dk0 + 2G adds); read whether or not the job exists: the index
-          // stays inside the table, and an absent job loads through the zero
-          // resource
-          const int oa = dk0 + 2 * G + W - 1;
+          // table entries of the next loads (read whether or not the job
+          // exists: the index stays inside the table)
 #pragma unroll
-          for (int q = 0; q < kJobs; ++q) t.mj[q] = tab(js[q], oa + jc[q]);
+          for (int q = 0; q < kJobs; ++q) t.mj[q] = tab(js[q], jo[q]);
         };
         // Steps dk0 + u, u < n (FULL: all G).  Every wave sweeps the
         // workgroup's offsets: each column's window covers all waves' reads up
@@ -1031,28 +1038,28 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         // loads whether or not its jobs exist (absent jobs read out of range),
         // so no loaded value is merged across a branch and nothing waits on
         // the loads before the stores.
-        // The new columns of group dk0 + G (oa = dk0 + 2G + W - 1... for the
-        // loads issued in group dk0) are loaded two groups ahead of their
-        // reads: issued at the start of group dk0 - G, held in registers
-        // through that group and group dk0, stored at the end of group dk0.
+        // The G columns a group adds to the ring are loaded two groups ahead
+        // of their reads: the loads issued at the start of group dk0 are held
+        // in registers through groups dk0 and dk0 + G, stored at the end of
+        // group dk0 + G and read from group dk0 + 2G on.
         using Jobs = V4[kJobs];
         using Dsts = int[kJobs];
-        auto issue = [&](int oa, const int* m, Jobs& v, Dsts& dst) {
+        auto issue = [&](const int* m, Jobs& v, Dsts& dst) {
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
             // a job loads pairs lo .. lo + kWinPairs - 1 of its column (lo * kB
-            // = slot - table entry); pairs past the trace end land in slot
+            // = slot - table entry m); pairs past the trace end land in slot
             // entries no read reaches (reads stop at hi <= n_t - 1) and come
             // from the next trace of the row, which holds at least kSlot bytes
-            const int o = oa + jc[q];
-            const int k = js[q] == 0 ? kbase + o : kbase + (W - 1) - o;
-            const bool ok = jh[q] && o < nO;
-            const bool data = ok && k >= xd_lo && k < xd_hi;
-            const int sl = PL::slot(js[q], o);
-            dst[q] = ok ? sl + jr[q] * PL::kRow : PL::kSpare;
-            v[q] = __builtin_amdgcn_raw_buffer_load_b128(data ? (jr[q] == 0 ? rs0 : rs1) : rsz,
-                                                         sl + lbs - m[q],
-                                                         data ? (k - xd_lo) * tb : 0, 0);
+            const bool data = (unsigned)jkd[q] < (unsigned)a.n_xd;
+            dst[q] = jo[q] < jlim[q] ? jsl[q] : PL::kSpare;
+            v[q] = __builtin_amdgcn_raw_buffer_load_b128(jrs[q], jsl[q] + jlbs[q] - m[q],
+                                                         data ? jso[q] : a.n_xd * tb, 0);
+            jo[q] += G;
+            jkd[q] += jdkd[q];
+            jso[q] += jdkd[q] * tb;
+            jsl[q] += G * PL::kSlot;
+            if (jsl[q] >= jslend[q]) jsl[q] -= PL::kRing * PL::kSlot;
           }
         };
         // one group: issue the loads for group dk0 + 2G (into vi), sum the
@@ -1060,7 +1067,7 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         // group earlier (vs) and make them visible.
         auto run_group = [&](auto nr_c, auto full_c, auto first_c, int dk0, Tab& t, Jobs& vs,
                              Dsts& ds, Jobs& vi, Dsts& di) {
-          issue(dk0 + 2 * G + W - 1, t.mj, vi, di);
+          issue(t.mj, vi, di);
           compute(nr_c, full_c, first_c, dk0, min(G, kmax_wg + 1 - dk0), t);
           read_tab(dk0 + G, t);
           // The stores wait on the loads, so the group's sum must be complete
@@ -1078,16 +1085,16 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         auto run_row = [&](auto nr_c) {
           using T = std::integral_constant<bool, true>;
           using F = std::integral_constant<bool, false>;
-          Tab t;
-          read_tab(0, t);
           Jobs va, vb;
           Dsts da, db;
           {  // the loads stored at the end of group 0
             int m0[kJobs];
 #pragma unroll
-            for (int q = 0; q < kJobs; ++q) m0[q] = tab(js[q], G + W - 1 + jc[q]);
-            issue(G + W - 1, m0, va, da);
+            for (int q = 0; q < kJobs; ++q) m0[q] = tab(js[q], jo[q]);
+            issue(m0, va, da);
           }
+          Tab t;
+          read_tab(0, t);
           __syncthreads();  // the first windows are visible
           if (kmax_wg + 1 >= G)
             run_group(nr_c, T{}, T{}, 0, t, va, da, vb, db);
