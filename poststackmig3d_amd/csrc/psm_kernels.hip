@@ -125,6 +125,7 @@ template <>
 struct Store<PSM_STORE_F32> {
   using Raw = float2;
   static constexpr int kBytes = 8;
+  static constexpr int kShift = 3;  // log2 kBytes
   static constexpr bool kScaled = false;
   __device__ static Raw load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
@@ -137,6 +138,7 @@ template <>
 struct Store<PSM_STORE_F16> {
   using Raw = unsigned;
   static constexpr int kBytes = 4;
+  static constexpr int kShift = 2;  // log2 kBytes
   static constexpr bool kScaled = true;
   __device__ static Raw load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
@@ -153,6 +155,7 @@ template <>
 struct Store<PSM_STORE_BF16> {
   using Raw = unsigned;
   static constexpr int kBytes = 4;
+  static constexpr int kShift = 2;  // log2 kBytes
   static constexpr bool kScaled = true;
   __device__ static Raw load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
@@ -680,7 +683,7 @@ struct LdsPlan {
 // addressed through a buffer resource with a 32-bit size).
 constexpr size_t kLdsMaxRowBytes = (size_t)1 << 30;
 
-template <int ST, int W, int G>
+template <int ST, int W, int G, bool AMP>
 __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n_tiles,
                                                                 int n_jgroups, int per_tile_pad) {
   using S = Store<ST>;
@@ -978,9 +981,14 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
               const float r2 = t.r2[u];
               const float tau = sqrt_up(__builtin_fmaf(p.a, r2, p.t0sq));
               const float q = tau * a.inv_dt;
-              const int ib = min((int)q, nt1) * kB;
-              const char* xr = smem + t.bR[u] + ib;
-              const char* xl = smem + t.bL[u] + ib;
+              // both column addresses as (i << log2 kB) + base, one
+              // instruction each (the compiler shares the shift otherwise)
+              const int i = min((int)q, nt1);
+              int ar, al;
+              asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(ar) : "v"(i), "i"(S::kShift), "v"(t.bR[u]));
+              asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(al) : "v"(i), "i"(S::kShift), "v"(t.bL[u]));
+              const char* xr = smem + ar;
+              const char* xl = smem + al;
               st[u].x[0] = *reinterpret_cast<const Raw*>(xr);
               st[u].x[1] = one ? Raw{} : *reinterpret_cast<const Raw*>(xl);
               if constexpr (NR == 2) {
@@ -999,7 +1007,7 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
                 }
               }
               st[u].f = __builtin_amdgcn_fractf(q);
-              st[u].wt = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+              st[u].wt = AMP ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
               st[u].m = r2 <= p.R2;
             }
           }
@@ -1363,8 +1371,12 @@ hipError_t launch_lds_wg(const MigArgs& a, hipStream_t stream) {
   const long long per_tile_pad = (per_tile + kXcds - 1) / kXcds * kXcds;
   const long long blocks = per_tile_pad * n_tiles;
   if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((migrate_lds_kernel<ST, W, G>), dim3((unsigned)blocks), dim3(kWave * W), 0,
-                     stream, a, n_tiles, n_jgroups, (int)per_tile_pad);
+  if (a.amp)
+    hipLaunchKernelGGL((migrate_lds_kernel<ST, W, G, true>), dim3((unsigned)blocks),
+                       dim3(kWave * W), 0, stream, a, n_tiles, n_jgroups, (int)per_tile_pad);
+  else
+    hipLaunchKernelGGL((migrate_lds_kernel<ST, W, G, false>), dim3((unsigned)blocks),
+                       dim3(kWave * W), 0, stream, a, n_tiles, n_jgroups, (int)per_tile_pad);
   return hipGetLastError();
 }
 
