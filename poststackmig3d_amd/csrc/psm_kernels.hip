@@ -714,6 +714,9 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
   if (in_out && in_buf) ls = lane_setup<kForward>(a, tk, io, j, kj, s, active);
   const bool valid = ls.src >= 0;
   const float R2w = uniform_f(wave_max(ls.L));
+  // smallest aperture R^2 of the wave's valid lanes (a lane without a valid
+  // sample needs no mask: its sum is discarded)
+  const float R2in = uniform_f(wave_min(valid ? ls.p.R2 : 3.0e38f));
   const float Tlo2 = uniform_f(wave_min(valid ? ls.p.t0sq : 3.0e38f));
   const float Thi2 = uniform_f(wave_max(valid ? ls.p.t0sq : 0.0f));
   const float Alo = uniform_f(wave_min(valid ? ls.p.a : 3.0e38f));
@@ -804,6 +807,20 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
       const int kmax_wg = uniform_i(wave_max_i(km));
       if (kmax_wg < 0) continue;
       const int nO = kmax_wg + W;  // column offsets per side
+      // last crossline offset inside every valid lane's aperture (r^2 as in
+      // the column table), at most kmax_wg; -1 if none
+      int kin = -1;
+      if (R2in >= d2) {
+        const float kf = __builtin_sqrtf(R2in - d2) * a.inv_dy;
+        int k = kf >= (float)kmax_wg ? kmax_wg : (int)kf + 1;
+        while (k >= 0) {
+          const float dyv = (float)k * a.dy;
+          if (__builtin_fmaf(dyv, dyv, d2) <= R2in) break;
+          --k;
+        }
+        kin = k;
+      }
+      kin = uniform_i(kin);
       const int r0 = hp ? rp : rm;  // ring row 0; ring row 1 = rm when both rows exist
       const bool two = hp && hm;
 
@@ -962,8 +979,10 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         // to kmax_wg, and a step past a wave's own last offset adds only masked
         // terms or exact zeros (pairs past the trace end).  FIRST: the group
         // holds dk = 0, whose -dk column is the +dk column and is left out.
-        auto compute = [&](auto nr_c, auto full_c, auto first_c, int dk0, int n, const Tab& t) {
+        auto compute = [&](auto nr_c, auto full_c, auto first_c, auto mask_c, int dk0, int n,
+                           const Tab& t) {
           constexpr int NR = decltype(nr_c)::value;
+          constexpr bool MASK = decltype(mask_c)::value;  // false: every lane is inside
           constexpr bool FULL = decltype(full_c)::value;
           constexpr bool FIRST = decltype(first_c)::value;
           struct Stp {
@@ -1008,7 +1027,7 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
               }
               st[u].f = __builtin_amdgcn_fractf(q);
               st[u].wt = AMP ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
-              st[u].m = r2 <= p.R2;
+              if constexpr (MASK) st[u].m = r2 <= p.R2;
             }
           }
           // slot order (row, column) = (+db,+dk), (+db,-dk), (-db,+dk), (-db,-dk)
@@ -1034,7 +1053,7 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
               } else {
                 sum = S::lerp(q.x[0], q.f) + S::lerp(q.x[1], q.f);
               }
-              sum = q.m ? sum : 0.0f;
+              if constexpr (MASK) sum = q.m ? sum : 0.0f;
               acc = __builtin_fmaf(sum, q.wt, acc);
             }
           }
@@ -1073,10 +1092,10 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         // one group: issue the loads for group dk0 + 2G (into vi), sum the
         // group, read the next group's tables, store the loads issued one
         // group earlier (vs) and make them visible.
-        auto run_group = [&](auto nr_c, auto full_c, auto first_c, int dk0, Tab& t, Jobs& vs,
-                             Dsts& ds, Jobs& vi, Dsts& di) {
+        auto run_group = [&](auto nr_c, auto full_c, auto first_c, auto mask_c, int dk0, Tab& t,
+                             Jobs& vs, Dsts& ds, Jobs& vi, Dsts& di) {
           issue(t.mj, vi, di);
-          compute(nr_c, full_c, first_c, dk0, min(G, kmax_wg + 1 - dk0), t);
+          compute(nr_c, full_c, first_c, mask_c, dk0, min(G, kmax_wg + 1 - dk0), t);
           read_tab(dk0 + G, t);
           // The stores wait on the loads, so the group's sum must be complete
           // before them: the empty asm takes the accumulator as an input, and
@@ -1105,20 +1124,26 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
           read_tab(0, t);
           __syncthreads();  // the first windows are visible
           if (kmax_wg + 1 >= G)
-            run_group(nr_c, T{}, T{}, 0, t, va, da, vb, db);
+            run_group(nr_c, T{}, T{}, T{}, 0, t, va, da, vb, db);
           else
-            run_group(nr_c, F{}, T{}, 0, t, va, da, vb, db);
-          // groups alternate the two register sets (unrolled by two)
+            run_group(nr_c, F{}, T{}, T{}, 0, t, va, da, vb, db);
+          // groups alternate the two register sets (unrolled by two); pairs
+          // of groups inside every lane's aperture (dk <= kin) skip the mask
+          // (the barriers are the same on both paths)
           int dk0 = G;
+          for (; dk0 + 2 * G <= min(kmax_wg, kin) + 1; dk0 += 2 * G) {
+            run_group(nr_c, T{}, F{}, F{}, dk0, t, vb, db, va, da);
+            run_group(nr_c, T{}, F{}, F{}, dk0 + G, t, va, da, vb, db);
+          }
           for (; dk0 + 2 * G <= kmax_wg + 1; dk0 += 2 * G) {
-            run_group(nr_c, T{}, F{}, dk0, t, vb, db, va, da);
-            run_group(nr_c, T{}, F{}, dk0 + G, t, va, da, vb, db);
+            run_group(nr_c, T{}, F{}, T{}, dk0, t, vb, db, va, da);
+            run_group(nr_c, T{}, F{}, T{}, dk0 + G, t, va, da, vb, db);
           }
           if (dk0 + G <= kmax_wg + 1) {
-            run_group(nr_c, T{}, F{}, dk0, t, vb, db, va, da);
-            if (dk0 + G <= kmax_wg) run_group(nr_c, F{}, F{}, dk0 + G, t, va, da, vb, db);
+            run_group(nr_c, T{}, F{}, T{}, dk0, t, vb, db, va, da);
+            if (dk0 + G <= kmax_wg) run_group(nr_c, F{}, F{}, T{}, dk0 + G, t, va, da, vb, db);
           } else if (dk0 <= kmax_wg) {
-            run_group(nr_c, F{}, F{}, dk0, t, vb, db, va, da);
+            run_group(nr_c, F{}, F{}, T{}, dk0, t, vb, db, va, da);
           }
         };
         if (two)
