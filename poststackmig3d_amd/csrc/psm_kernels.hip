@@ -1075,12 +1075,13 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
             // a job loads pairs lo .. lo + kWinPairs - 1 of its column (lo * kB
-            // = slot - table entry m); pairs past the trace end land in slot
-            // entries no read reaches (reads stop at hi <= n_t - 1) and come
-            // from the next trace of the row, which holds at least kSlot bytes
+            // = slot - table entry m); lanes whose pairs all lie past the
+            // trace end load out of range (no read reaches those slot
+            // entries: reads stop at hi <= n_t - 1)
             const bool data = (unsigned)jkd[q] < (unsigned)a.n_xd;
             dst[q] = jo[q] < jlim[q] ? jsl[q] : PL::kSpare;
-            v[q] = __builtin_amdgcn_raw_buffer_load_b128(jrs[q], jsl[q] + jlbs[q] - m[q],
+            const int voff = jsl[q] + jlbs[q] - m[q];  // lo * kB + lane * 16
+            v[q] = __builtin_amdgcn_raw_buffer_load_b128(jrs[q], voff > nt1 * kB ? 0x7ff00000 : voff,
                                                          data ? jso[q] : a.n_xd * tb, 0);
             jo[q] += G;
             jkd[q] += jdkd[q];
