@@ -399,14 +399,15 @@ def main():
                                              (args.kernel == "auto" and args.storage == "f32"))
                     else "migrate_kernel", "32" if args.storage == "f32" else "16"),
                 "kernel_ms": round(avg_kernel_s * 1e3, 3),
-                "kernel_ms_is": "sum of the step's migrate_kernel launches (aperture-row passes), "
-                                "HIP events on the launch stream",
+                "kernel_ms_is": "sum of the step's forward-kernel launches (one launch unless "
+                                "aperture-row passes are set), HIP events on the launch stream",
                 "algorithmic_bytes": alg_bytes,
                 "rocprof_counters": counters or None,
                 "traffic_gbs": round(traffic / avg_kernel_s / 1e9, 1) if traffic else None,
                 "note": f"algorithmic bytes = {pair_bytes:.0f} B per gathering pair (P_g); the kernel reuses "
-                        "each trace window from L1/L2 across neighbouring output traces, so "
-                        "HBM traffic is far below this and frac > 1 is possible"}
+                        "each trace window across neighbouring output traces (from LDS in the "
+                        "staged form, L1/L2 in the gather form), so HBM traffic is far below "
+                        "this and frac > 1 is possible"}
 
     variants = {}
     if not args.no_variants and args.storage == "f32":
