@@ -869,12 +869,13 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
         // ---- staging jobs: job j of a set of columns [oa, oa + nc) is
         // (column oa + j / (2 nr), side (j / nr) % 2, ring row j % nr), nr =
         // rows of the pair; wave w takes jobs w, w + W, ...  A job loads the
-        // column's window of one row into registers, 16 bytes per lane
-        // (load_job), and writes it to the column's ring slot (store_job), so
-        // LDS reads never wait on loads in flight.  Columns outside the data
-        // and samples past the trace end are loaded out of range (zeros).
+        // column's window of one row, 16 bytes per lane: the first columns of
+        // a row straight into LDS, later ones into registers that are written
+        // to the column's ring slot at the end of a group, so LDS reads never
+        // wait on loads in flight.  Pairs past the trace end are loaded out
+        // of range; columns outside the data are zeros (out of range in the
+        // first fill, the row's zero trace afterwards).
         using V4 = unsigned __attribute__((ext_vector_type(4)));
-        const int nr = two ? 2 : 1;
         const int lb = lane * 16;
         const int lbs = lb < PL::kSlot ? lb : 0x7ff00000;  // lanes past the slot load nothing
         const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
