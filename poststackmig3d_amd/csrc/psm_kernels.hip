@@ -1078,7 +1078,9 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
             // a job loads pairs lo .. lo + kWinPairs - 1 of its column (lo * kB
             // = slot - table entry m); lanes whose pairs all lie past the
             // trace end load out of range (no read reaches those slot
-            // entries: reads stop at hi <= n_t - 1)
+            // entries: reads stop at hi <= n_t - 1), the lane holding pair
+            // n_t - 1 reads at most 12 bytes into the next trace of the row
+            // (every row ends with a zero trace)
             const bool data = (unsigned)jkd[q] < (unsigned)a.n_xd;
             dst[q] = jo[q] < jlim[q] ? jsl[q] : PL::kSpare;
             const int voff = jsl[q] + jlbs[q] - m[q];  // lo * kB + lane * 16
@@ -1435,10 +1437,7 @@ hipError_t launch_migrate(const MigArgs& a0, int mode, hipStream_t stream, const
   // LDS-staged forward kernel: contiguous rows, 4 or 8 output crosslines per
   // workgroup, rows addressable by a 32-bit buffer size
   const size_t row_bytes = (size_t)(a0.n_xd + 1) * a0.n_t * (a0.storage == PSM_STORE_F32 ? 8 : 4);
-  // (a staging load reads up to kWinPairs pairs from its window start, which
-  // can pass the trace end by that much: traces must be at least that long)
-  const bool lds_ok = mode == kForward && a0.row_pos == nullptr && row_bytes <= kLdsMaxRowBytes &&
-                      a0.n_t >= kWinPairs;
+  const bool lds_ok = mode == kForward && a0.row_pos == nullptr && row_bytes <= kLdsMaxRowBytes;
   const bool auto_lds = tune.staging < 0 && a0.storage == PSM_STORE_F32 && lds_ok;
   const int wpb = auto_lds ? kAutoStageWaves : tune.waves_per_block;
   const int steps = auto_lds ? kAutoStageSteps : tune.stage_steps;

@@ -7,7 +7,8 @@ reference kernel to the parity bar (1e-5).  The cases reach every path of the
 staged kernel: sticky trailing crosslines (the workgroup runs the gather
 path), coarse crossline bins with a slow velocity (time windows wider than a
 staged slot: per-row fallback), fine crossline bins (more column offsets than
-the column table holds: per-row fallback), and both storage forms.
+the column table holds: per-row fallback), traces shorter than a slot, and
+both storage forms.
 """
 import numpy as np
 import pytest
@@ -51,6 +52,15 @@ def _case(name):
         data, _ = synth.point_diffractor(*n, dy=0.5, noise=0.1, seed=6)
         vel = np.full(n, 2000.0, np.float32)
         geo = dict(inline_step=25.0, crossline_step=0.5, angle_aperture=60.0)
+    elif name == "short_traces":
+        # 40 samples: every window reaches the trace end, and the staging
+        # loads stop at it (the lane holding the last pair reads into the next
+        # trace of the row)
+        n = (21, 23, 40)
+        data, _ = synth.production(*n, n_diffractors=3, seed=9)
+        vel = synth.production_velocity(*n)
+        geo = dict(inline_step=25.0, crossline_step=25.0, inline_padding=2, crossline_padding=3,
+                   angle_aperture=60.0)
     else:
         raise KeyError(name)
     prm = P.SurveyParams.for_cube(n, **geo)
@@ -59,7 +69,7 @@ def _case(name):
     return data, vel, prm, ref_args
 
 
-CASE_NAMES = ["padded_lateral", "point_unpadded", "coarse_slow", "fine_bins"]
+CASE_NAMES = ["padded_lateral", "point_unpadded", "coarse_slow", "fine_bins", "short_traces"]
 
 
 @pytest.fixture(scope="module")
