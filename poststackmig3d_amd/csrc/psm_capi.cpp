@@ -728,6 +728,9 @@ int psm_last_timing(psm_ctx* c, psm_timing* t) {
 }  // extern "C"
 
 // ---------------------------------------------------------------- batches
+// Resident workgroups per CU of the staged batch form (its LDS: three 8-wave
+// workgroups of ~52 KB in 160 KB).
+constexpr int kBatchStagedPerCu = 3;
 // Several volumes (a velocity scan over scaled Vrms models, sub-volumes of
 // any size) migrated by one persistent launch, captured once into a HIP graph
 // together with the trace packing and the queue reset, then replayed.
@@ -737,6 +740,7 @@ struct psm_batch {
   int storage = PSM_STORE_F32;
   int n_items = 0;
   int wpb = 4, blocks_per_cu = 0;
+  bool staged = false;  // kernel form of the captured launch
   int n_cu = 0;
   std::vector<psm::MigArgs> args;
   std::vector<psm::BatchSeg> segs;
@@ -770,7 +774,9 @@ int batch_capture(psm_batch* b) {
   psm_ctx* c = b->ctx;
   batch_release_graph(b);
   hipStream_t st = c->own_stream;  // capture needs a stream of our own
-  const int blocks = b->n_cu * (b->blocks_per_cu > 0 ? b->blocks_per_cu : std::max(1, 32 / b->wpb));
+  const int blocks = b->n_cu * (b->blocks_per_cu > 0 ? b->blocks_per_cu
+                                : b->staged     ? kBatchStagedPerCu
+                                                : std::max(1, 32 / b->wpb));
   PSM_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   hipError_t e = psm::launch_fill_zero(b->counter.as<float>(), 1, st);
   for (auto& g : b->groups) {
@@ -779,7 +785,8 @@ int batch_capture(psm_batch* b) {
                          g->n_t, st);
   }
   if (e == hipSuccess)
-    e = psm::launch_batch(b->storage, b->d_args.as<psm::MigArgs>(), b->d_segs.as<psm::BatchSeg>(),
+    e = psm::launch_batch(b->storage, b->staged, b->d_args.as<psm::MigArgs>(),
+                          b->d_segs.as<psm::BatchSeg>(),
                           (int)b->segs.size(), b->counter.as<unsigned>(), b->total, b->wpb, blocks,
                           st);
   hipGraph_t g = nullptr;
@@ -795,12 +802,17 @@ int batch_capture(psm_batch* b) {
 }
 
 // Builds the work queue for workgroups of wpb waves (latest t0 tiles of all
-// volumes first), uploads it and captures the graph.
+// volumes first), uploads it and captures the graph.  The kernel form follows
+// the context's staging switch as psm_migrate_volume does: the staged form
+// (workgroups of kAutoStageWaves crosslines; wpb only validated) for FP32
+// traces by default or for any storage when forced on, else the gather form.
 int batch_set_shape(psm_batch* b, int wpb, int blocks_per_cu) {
   psm_ctx* c = b->ctx;
   if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8)
     return fail(c, PSM_ERR_ARG, "batch waves_per_block must be 1, 2, 4 or 8");
   if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(c, PSM_ERR_ARG, "bad blocks_per_cu");
+  b->staged = c->tune.staging > 0 || (c->tune.staging < 0 && b->storage == PSM_STORE_F32);
+  if (b->staged) wpb = psm::kAutoStageWaves;
   b->wpb = wpb;
   b->blocks_per_cu = blocks_per_cu;
   b->segs.clear();
@@ -959,11 +971,17 @@ int psm_batch_autotune(psm_batch* b, int* waves_per_block, int* blocks_per_cu, f
   psm_ctx* c = b->ctx;
   PSM_HIP(c, hipSetDevice(c->device));
   // candidate launch shapes: workgroup width x resident workgroups per CU
-  const int cand[][2] = {{1, 32}, {2, 16}, {4, 8}, {8, 4}, {2, 8}, {4, 4}};
+  // (the staged form: its width, up to the workgroups its LDS lets a CU hold)
+  const int cand_gather[][2] = {{1, 32}, {2, 16}, {4, 8}, {8, 4}, {2, 8}, {4, 4}};
+  const int cand_staged[][2] = {{psm::kAutoStageWaves, kBatchStagedPerCu},
+                                {psm::kAutoStageWaves, kBatchStagedPerCu - 1}};
+  const bool staged = c->tune.staging > 0 || (c->tune.staging < 0 && b->storage == PSM_STORE_F32);
+  const int n_cand = staged ? 2 : 6;
   float best = -1.0f;
   int bw = 4, bb = 0;
   bool warm = false;
-  for (const auto& cd : cand) {
+  for (int ci = 0; ci < n_cand; ++ci) {
+    const int* cd = staged ? cand_staged[ci] : cand_gather[ci];
     int rc = batch_set_shape(b, cd[0], cd[1]);
     if (rc) return rc;
     float ms = 0.0f;

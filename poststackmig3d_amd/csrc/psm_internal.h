@@ -110,8 +110,12 @@ hipError_t launch_table_velocity(const psm_velocity_point* pts, int n_pts, float
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream);
 // Persistent batch launch (forward migration of several volumes): `blocks`
 // workgroups of wpb waves drain `total` queue positions described by segs.
-hipError_t launch_batch(int storage, const MigArgs* items, const BatchSeg* segs, int n_segs,
-                        unsigned* counter, unsigned total, int wpb, int blocks, hipStream_t stream);
+// staged: workgroups of the staged form (kAutoStageWaves output crosslines,
+// kAutoStageSteps offsets per group; items of that width, wpb unused), else
+// the gather form with wpb waves per workgroup.
+hipError_t launch_batch(int storage, bool staged, const MigArgs* items, const BatchSeg* segs,
+                        int n_segs, unsigned* counter, unsigned total, int wpb, int blocks,
+                        hipStream_t stream);
 // out[0] = sum x[i] y[i] in float64; partial holds dot_scratch_doubles() - 1 doubles.
 size_t dot_scratch_doubles();
 hipError_t launch_dot_f64(const float* x, const float* y, size_t n, double* partial, double* out,

@@ -115,6 +115,19 @@ __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfi
 // occupying the vector memory path.
 using const_f32_ptr = const __attribute__((address_space(4))) float*;
 __device__ __forceinline__ const_f32_ptr as_const(const float* p) { return (const_f32_ptr)p; }
+// A whole struct read that way (dword by dword: the implicit copy does not
+// bind across address spaces).
+template <class T>
+__device__ __forceinline__ T const_load(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "dword-sized struct");
+  using CU = const __attribute__((address_space(4))) unsigned*;
+  T v;
+  unsigned* d = reinterpret_cast<unsigned*>(&v);
+  const CU q = (CU)p;
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) d[k] = q[k];
+  return v;
+}
 
 // ---------------------------------------------------------------- storage
 // One stored pair (x[i], x[i+1] - x[i]) and its linear interpolation in fp32.
@@ -674,6 +687,9 @@ struct LdsPlan {
   static constexpr int kPar = kR2 + kCols * 4;           // float par[5][W]
   static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
   static constexpr int kBytes = kFlag + W * 4;
+  // waves per SIMD the LDS allows (three 8-wave workgroups per CU with the
+  // double ring), else no register target
+  static constexpr int kWavesPerEu = (W == 8 && !kSingle && 3 * (kBytes + 16) <= 160 * 1024) ? 6 : 1;
   static_assert(kRow <= 65535, "ring row offset must fit a ds_read offset");
   static_assert(kSlot <= 16 * kWave, "one load per window");
   __device__ static int slot(int side, int o) { return side * kSide + (o % kRing) * kSlot; }
@@ -683,23 +699,16 @@ struct LdsPlan {
 // addressed through a buffer resource with a 32-bit size).
 constexpr size_t kLdsMaxRowBytes = (size_t)1 << 30;
 
+// One workgroup of the staged form: output crosslines j0 .. j0 + W - 1 of
+// output inline io over t0 tile `tile`; smem holds LdsPlan<ST, W, G>::kBytes
+// of LDS.  Every wave of the workgroup calls it with the same arguments.
 template <int ST, int W, int G, bool AMP>
-__global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n_tiles,
-                                                                int n_jgroups, int per_tile_pad) {
+__device__ __forceinline__ void staged_tile(const MigArgs& a, int tile, int io, int j0,
+                                            char* smem) {
   using S = Store<ST>;
   using Raw = typename S::Raw;
   using PL = LdsPlan<ST, W, G>;
   constexpr int kB = S::kBytes;
-  __shared__ __attribute__((aligned(16))) char smem[PL::kBytes];
-
-  // ---- block -> (t0 tile, output inline, crossline group), as migrate_kernel
-  const int g = blockIdx.x;
-  const int tile = n_tiles - 1 - g / per_tile_pad;
-  const int rem = g % per_tile_pad;
-  const int spatial = (rem % kXcds) * (per_tile_pad / kXcds) + rem / kXcds;
-  if (spatial >= a.n_out_il * n_jgroups) return;  // whole workgroup
-  const int io = spatial / n_jgroups;
-  const int j0 = (spatial % n_jgroups) * W;
   const int w = uniform_i((int)(threadIdx.x / kWave));
   const int lane = threadIdx.x % kWave;
   const int j = j0 + w;
@@ -708,7 +717,7 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
   const bool active = s < a.n_t;
   const bool in_out = j < a.n_out_xl;
   const bool in_buf = kj >= 0 && kj < a.K;
-  const Task tk = a.tasks[io];
+  const Task tk = const_load<Task>(a.tasks + io);  // scalar, as the row indices derive from it
 
   LaneSetup ls{{0.0f, 0.0f, -1.0f, 1.0f, 0, 0, 0.0f}, -1, false, -1.0f};
   if (in_out && in_buf) ls = lane_setup<kForward>(a, tk, io, j, kj, s, active);
@@ -1177,6 +1186,21 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
   }
 }
 
+// The staged form over a grid of (t0 tile, output inline, crossline group)
+// workgroups: latest tiles first, consecutive spatial indices of a tile on one
+// XCD (as migrate_kernel).
+template <int ST, int W, int G, bool AMP>
+__global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n_tiles,
+                                                                int n_jgroups, int per_tile_pad) {
+  __shared__ __attribute__((aligned(16))) char smem[LdsPlan<ST, W, G>::kBytes];
+  const int g = blockIdx.x;
+  const int tile = n_tiles - 1 - g / per_tile_pad;
+  const int rem = g % per_tile_pad;
+  const int spatial = (rem % kXcds) * (per_tile_pad / kXcds) + rem / kXcds;
+  if (spatial >= a.n_out_il * n_jgroups) return;  // whole workgroup
+  staged_tile<ST, W, G, AMP>(a, tile, spatial / n_jgroups, (spatial % n_jgroups) * W, smem);
+}
+
 // Persistent batch kernel: every workgroup takes work items (one
 // (volume, t0 tile, crossline group) each) from a global counter until the
 // queue is drained, so volumes of very different size and cost share one
@@ -1193,7 +1217,9 @@ __global__ __launch_bounds__(kWave * WPB) void batch_kernel(const MigArgs* __res
   for (;;) {
     if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
     __syncthreads();
-    const unsigned w = s_next;
+    // (read from LDS, so made uniform explicitly: everything derived from it
+    // -- the item's arguments, its buffer resources -- stays scalar)
+    const unsigned w = (unsigned)uniform_i((int)s_next);
     __syncthreads();
     if (w >= total) return;
     int lo = 0, hi = n_segs - 1;
@@ -1207,6 +1233,46 @@ __global__ __launch_bounds__(kWave * WPB) void batch_kernel(const MigArgs* __res
     const int io = spatial / sg.n_jgroups;
     const int j = (spatial % sg.n_jgroups) * WPB + uniform_i((int)(threadIdx.x / kWave));
     if (io < a.n_out_il && j < a.n_out_xl) migrate_trace_tile<ST, false, kForward>(a, sg.tile, io, j);
+  }
+}
+
+// The same queue over workgroups of the staged form (W output crosslines per
+// item): each item runs staged_tile, so its result equals migrate_lds_kernel's
+// for that (tile, inline, crossline group) bit for bit.  The barriers around
+// the queue read also separate one item's LDS use from the next.
+// (registers capped for the residency the LDS allows: the item's scalar
+// state otherwise spills into VGPR lanes past that)
+template <int ST, int W, int G>
+__global__ __launch_bounds__(kWave * W)
+__attribute__((amdgpu_waves_per_eu(LdsPlan<ST, W, G>::kWavesPerEu)))
+void batch_lds_kernel(const MigArgs* __restrict__ items, const BatchSeg* __restrict__ segs,
+                      int n_segs, unsigned* __restrict__ counter, unsigned total) {
+  __shared__ __attribute__((aligned(16))) char smem[LdsPlan<ST, W, G>::kBytes];
+  __shared__ unsigned s_next;
+  for (;;) {
+    if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
+    __syncthreads();
+    const unsigned wi = (unsigned)uniform_i((int)s_next);  // uniform (see batch_kernel)
+    __syncthreads();
+    if (wi >= total) return;
+    // the queue and the item's arguments through the constant address space:
+    // scalar loads, so the item's buffer resources stay in scalar registers
+    int lo = 0, hi = n_segs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if ((unsigned)const_load<BatchSeg>(segs + mid).first <= wi) lo = mid; else hi = mid - 1;
+    }
+    const BatchSeg sg = const_load<BatchSeg>(segs + lo);
+    const int spatial = (int)wi - sg.first;
+    const MigArgs a = const_load<MigArgs>(items + sg.item);
+    const int io = spatial / sg.n_jgroups;
+    const int j0 = (spatial % sg.n_jgroups) * W;
+    if (io < a.n_out_il) {
+      if (a.amp)
+        staged_tile<ST, W, G, true>(a, sg.tile, io, j0, smem);
+      else
+        staged_tile<ST, W, G, false>(a, sg.tile, io, j0, smem);
+    }
   }
 }
 
@@ -1485,9 +1551,16 @@ hipError_t launch_dot_f64(const float* x, const float* y, size_t n, double* part
 size_t dot_scratch_doubles() { return kDotBlocks + 1; }
 
 template <int ST>
-hipError_t launch_batch_st(const MigArgs* items, const BatchSeg* segs, int n_segs, unsigned* counter,
-                           unsigned total, int wpb, int blocks, hipStream_t stream) {
+hipError_t launch_batch_st(bool staged, const MigArgs* items, const BatchSeg* segs, int n_segs,
+                           unsigned* counter, unsigned total, int wpb, int blocks,
+                           hipStream_t stream) {
   const dim3 grid((unsigned)blocks);
+  if (staged) {
+    hipLaunchKernelGGL((batch_lds_kernel<ST, kAutoStageWaves, kAutoStageSteps>), grid,
+                       dim3(kWave * kAutoStageWaves), 0, stream, items, segs, n_segs, counter,
+                       total);
+    return hipGetLastError();
+  }
   switch (wpb) {
     case 1: hipLaunchKernelGGL((batch_kernel<ST, 1>), grid, dim3(kWave), 0, stream, items, segs, n_segs, counter, total); break;
     case 2: hipLaunchKernelGGL((batch_kernel<ST, 2>), grid, dim3(2 * kWave), 0, stream, items, segs, n_segs, counter, total); break;
@@ -1497,15 +1570,15 @@ hipError_t launch_batch_st(const MigArgs* items, const BatchSeg* segs, int n_seg
   return hipGetLastError();
 }
 
-hipError_t launch_batch(int storage, const MigArgs* items, const BatchSeg* segs, int n_segs,
-                        unsigned* counter, unsigned total, int wpb, int blocks, hipStream_t stream) {
+hipError_t launch_batch(int storage, bool staged, const MigArgs* items, const BatchSeg* segs,
+                        int n_segs, unsigned* counter, unsigned total, int wpb, int blocks,
+                        hipStream_t stream) {
   switch (storage) {
-    case PSM_STORE_F16: return launch_batch_st<PSM_STORE_F16>(items, segs, n_segs, counter, total, wpb, blocks, stream);
-    case PSM_STORE_BF16: return launch_batch_st<PSM_STORE_BF16>(items, segs, n_segs, counter, total, wpb, blocks, stream);
-    default: return launch_batch_st<PSM_STORE_F32>(items, segs, n_segs, counter, total, wpb, blocks, stream);
+    case PSM_STORE_F16: return launch_batch_st<PSM_STORE_F16>(staged, items, segs, n_segs, counter, total, wpb, blocks, stream);
+    case PSM_STORE_BF16: return launch_batch_st<PSM_STORE_BF16>(staged, items, segs, n_segs, counter, total, wpb, blocks, stream);
+    default: return launch_batch_st<PSM_STORE_F32>(staged, items, segs, n_segs, counter, total, wpb, blocks, stream);
   }
 }
-
 hipError_t launch_table_velocity(const psm_velocity_point* pts, int n_pts, float vmin,
                                  const int32_t* il, const int32_t* xl, int n_il, int n_xl,
                                  int n_t, double dt, float* vel, hipStream_t stream) {

@@ -1,9 +1,11 @@
 """Batched, graph-captured persistent migration: a velocity scan over scaled
 Vrms models plus uneven sub-volumes in one launch.  Each output must equal
 the single-volume migration of that item at the same velocity scale by the
-gather form of the forward kernel in one launch (the same per-wave arithmetic
-and summation order, so bit for bit), on every replay of the graph with eager
-work in between, and after autotuning."""
+same form of the forward kernel in one launch (the same per-workgroup
+arithmetic and summation order, so bit for bit), on every replay of the graph
+with eager work in between, and after autotuning.  The batch takes its form
+from the context's staging switch like psm_migrate_volume: staged for FP32
+traces by default, gather for FP16, gather for FP32 with staging off."""
 import numpy as np
 import pytest
 
@@ -35,12 +37,15 @@ def _items(torch, storage):
     return items
 
 
-def _reference(items, storage, wpb):
+def _reference(items, storage, wpb, staged):
     ctx = Context(0)
     try:
         ctx.set_storage(storage)
-        ctx.set_tuning(wpb, 0)
-        ctx.set_staging(False)
+        if staged:
+            ctx.set_staging(None)  # the automatic form: staged, 8 crosslines, one launch
+        else:
+            ctx.set_tuning(wpb, 0)
+            ctx.set_staging(False)
         refs = []
         for prm, d, v, out, alpha in items:
             ctx.set_velocity_scale(alpha, alpha)
@@ -50,13 +55,16 @@ def _reference(items, storage, wpb):
         ctx.close()
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_batch_matches_single_volumes(storage):
+@pytest.mark.parametrize("storage,staging", [("f32", None), ("f16", None), ("f32", False)])
+def test_batch_matches_single_volumes(storage, staging):
     import torch
     from poststackmig3d_amd.batch import Batch
     items = _items(torch, storage)
-    refs = _reference(items, storage, 4)
-    b = Batch(items, storage=storage)
+    staged = storage == "f32" and staging is None
+    refs = _reference(items, storage, 4, staged)
+    bctx = Context(0)
+    bctx.set_staging(staging)
+    b = Batch(items, storage=storage, ctx=bctx)
     try:
         for replay in range(3):
             for _, _, _, out, _ in items:
@@ -70,9 +78,11 @@ def test_batch_matches_single_volumes(storage):
             P.migrate_volume(items[0][1], items[0][2], items[0][0])
         w, bpc, ms = b.autotune()
         assert w in (1, 2, 4, 8) and ms > 0
+        assert not staged or w == 8
         b.run()
         b.ctx.synchronize()
         for k, (_, _, _, out, _) in enumerate(items):
             assert np.array_equal(out.cpu().numpy(), refs[k])
     finally:
         b.close()
+        bctx.close()
