@@ -178,9 +178,13 @@ int psm_migrate_volume_dalpha(psm_ctx* ctx, const psm_survey* survey,
  * context's stream (ms != NULL: waits and returns the elapsed time).
  * All item pointers are device memory and must stay valid while the batch
  * lives.  Item k's output equals psm_migrate_volume of that item at velocity
- * scale (alpha, alpha).  psm_batch_autotune times candidate launch shapes
- * (workgroup width x resident workgroups per CU) on the batch itself and keeps
- * the fastest.  Storage: PSM_STORE_* for all items.
+ * scale (alpha, alpha) by the same kernel form: the form follows the
+ * context's psm_set_staging setting when the shape is set (create, set_shape,
+ * autotune) -- staged for FP32 traces by default (workgroup width 8; the
+ * waves_per_block argument is then only validated), gather otherwise.
+ * psm_batch_autotune times candidate launch shapes (workgroup width x
+ * resident workgroups per CU) on the batch itself and keeps the fastest.
+ * Storage: PSM_STORE_* for all items.
  */
 typedef struct psm_batch psm_batch;
 typedef struct psm_batch_item {
