@@ -234,8 +234,10 @@ int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
  * per staged group in one launch, and gathers FP16/BF16 traces.  steps (2 or
  * 4) = crossline offsets per staged group when on = 1.  The staged form
  * applies to whole-volume calls and per-inline calls with contiguous rows;
- * other calls use the gather form.  Results do not depend on the form beyond
- * FP32 summation order. */
+ * other calls use the gather form.  The switch also selects the adjoint's
+ * form (psm_demigrate_volume): staged (LDS accumulator windows, 8 crosslines
+ * x 2 offsets) unless on = 0.  Results do not depend on the form beyond FP32
+ * summation order. */
 int psm_set_staging(psm_ctx* ctx, int on, int steps);
 int psm_get_staging(psm_ctx* ctx, int* on, int* steps);
 

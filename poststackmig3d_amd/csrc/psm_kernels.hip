@@ -666,12 +666,16 @@ struct StageTab {
   int mj[J];
 };
 
-template <int ST, int W, int G>
+// LDS of one staged workgroup.  Forward: slots of kWinPairs stored pairs.
+// Adjoint: slots of kWinPairs FP32 accumulators (data samples lo .. lo +
+// kWinPairs - 1) and a table of the samples each column flushes.
+template <int ST, int W, int G, int MODE = kForward>
 struct LdsPlan {
-  static constexpr int kB = Store<ST>::kBytes;
+  static constexpr bool kAdj = MODE == kAdjoint;
+  static constexpr int kB = kAdj ? 4 : Store<ST>::kBytes;  // bytes per pair / cell
   static constexpr int kSlot = kWinPairs * kB;           // bytes of one window
   static constexpr int kCols = kMaxCols + 4 * G + W;     // table entries (reads past nO stay inside)
-  static constexpr int kTables = 3 * kCols * 4 + 5 * W * 4 + W * 4;
+  static constexpr int kTables = (kAdj ? 5 : 3) * kCols * 4 + 5 * W * 4 + W * 4;
   // Double ring (2G + W - 1 slots per ring row and side): the next group's
   // windows are stored while the current group is read, one barrier per
   // group.  Single ring (G + W - 1 slots): they are stored after a barrier
@@ -686,12 +690,14 @@ struct LdsPlan {
   static constexpr int kR2 = kMeta + 2 * kCols * 4;      // float r2tab[kCols]
   static constexpr int kPar = kR2 + kCols * 4;           // float par[5][W]
   static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
-  static constexpr int kBytes = kFlag + W * 4;
+  static constexpr int kWid = kFlag + W * 4;             // adjoint: int wid[2][kCols]
+  static constexpr int kBytes = kWid + (kAdj ? 2 * kCols * 4 : 0);
   // waves per SIMD the LDS allows (three 8-wave workgroups per CU with the
   // double ring), else no register target
   static constexpr int kWavesPerEu = (W == 8 && !kSingle && 3 * (kBytes + 16) <= 160 * 1024) ? 6 : 1;
   static_assert(kRow <= 65535, "ring row offset must fit a ds_read offset");
   static_assert(kSlot <= 16 * kWave, "one load per window");
+  static_assert(!kAdj || !kSingle, "the adjoint uses the double ring");
   __device__ static int slot(int side, int o) { return side * kSide + (o % kRing) * kSlot; }
 };
 
@@ -700,15 +706,18 @@ struct LdsPlan {
 constexpr size_t kLdsMaxRowBytes = (size_t)1 << 30;
 
 // One workgroup of the staged form: output crosslines j0 .. j0 + W - 1 of
-// output inline io over t0 tile `tile`; smem holds LdsPlan<ST, W, G>::kBytes
-// of LDS.  Every wave of the workgroup calls it with the same arguments.
-template <int ST, int W, int G, bool AMP>
+// output inline io over t0 tile `tile`; smem holds LdsPlan<ST, W, G,
+// MODE>::kBytes of LDS.  Every wave of the workgroup calls it with the same
+// arguments.  MODE = kForward sums the staged windows into the output;
+// kAdjoint scatters the image samples into accumulator windows that are
+// flushed into the data cube (ST = F32).
+template <int ST, int W, int G, bool AMP, int MODE = kForward>
 __device__ __forceinline__ void staged_tile(const MigArgs& a, int tile, int io, int j0,
                                             char* smem) {
   using S = Store<ST>;
   using Raw = typename S::Raw;
-  using PL = LdsPlan<ST, W, G>;
-  constexpr int kB = S::kBytes;
+  using PL = LdsPlan<ST, W, G, MODE>;
+  constexpr int kB = PL::kB;
   const int w = uniform_i((int)(threadIdx.x / kWave));
   const int lane = threadIdx.x % kWave;
   const int j = j0 + w;
@@ -720,7 +729,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs& a, int tile, int io, 
   const Task tk = const_load<Task>(a.tasks + io);  // scalar, as the row indices derive from it
 
   LaneSetup ls{{0.0f, 0.0f, -1.0f, 1.0f, 0, 0, 0.0f}, -1, false, -1.0f};
-  if (in_out && in_buf) ls = lane_setup<kForward>(a, tk, io, j, kj, s, active);
+  if (in_out && in_buf) ls = lane_setup<MODE>(a, tk, io, j, kj, s, active);
   const bool valid = ls.src >= 0;
   const float R2w = uniform_f(wave_max(ls.L));
   // smallest aperture R^2 of the wave's valid lanes (a lane without a valid
@@ -766,9 +775,9 @@ __device__ __forceinline__ void staged_tile(const MigArgs& a, int tile, int io, 
     // sticky crossline windows in the group: every wave runs the gather path
     if (R2w >= 0.0f) {
       if (win)
-        sum_aperture<ST, false, true, kForward>(a, tk, kj, ls.p, R2w, acc, cap, cg);
+        sum_aperture<ST, false, true, MODE>(a, tk, kj, ls.p, R2w, acc, cap, cg);
       else
-        sum_aperture<ST, false, false, kForward>(a, tk, kj, ls.p, R2w, acc, cap, cg);
+        sum_aperture<ST, false, false, MODE>(a, tk, kj, ls.p, R2w, acc, cap, cg);
     }
   } else if (R2wg >= 0.0f) {
     const int c = tk.c;
@@ -860,7 +869,13 @@ __device__ __forceinline__ void staged_tile(const MigArgs& a, int tile, int io, 
                 // one index of slack each way covers the rounding of q
                 lo = min(max((int)(lo_t * a.inv_dt) - 1, 0), nt1);
                 const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
-                ovf = ovf || (hi - lo >= kWinPairs);
+                // (the adjoint also touches sample hi + 1)
+                ovf = ovf || (hi - lo >= (PL::kAdj ? kWinPairs - 1 : kWinPairs));
+                if constexpr (PL::kAdj)
+                  reinterpret_cast<int*>(smem + PL::kWid)[side * PL::kCols + o] =
+                      min(hi + 1, nt1) - lo + 1;  // samples the column flushes
+              } else if constexpr (PL::kAdj) {
+                reinterpret_cast<int*>(smem + PL::kWid)[side * PL::kCols + o] = 0;
               }
               meta[side * PL::kCols + o] = PL::slot(side, o) - lo * kB;
             }
@@ -870,11 +885,149 @@ __device__ __forceinline__ void staged_tile(const MigArgs& a, int tile, int io, 
           const float dyv = (float)dk * a.dy;
           r2tab[dk] = __builtin_fmaf(dyv, dyv, d2);
         }
+        if constexpr (PL::kAdj) {
+          // the accumulators of both ring rows start at zero
+          using Z4 = unsigned __attribute__((ext_vector_type(4)));
+          for (int off = (int)threadIdx.x * 16; off < 2 * PL::kRow; off += W * kWave * 16)
+            *reinterpret_cast<Z4*>(smem + off) = Z4{0u, 0u, 0u, 0u};
+        }
         if (lane == 0) flag[w] = __any(ovf) ? 1 : 0;
         __syncthreads();  // tables and flags are visible
         fallback = __any(pl && flag[pl ? lane : 0] != 0);
       }
-      if (!fallback) {
+      if constexpr (PL::kAdj) {
+        if (!fallback) {
+          // ---- adjoint: a column's slot accumulates the data samples
+          // lo .. lo + kWinPairs - 1 of its trace (lo as in the table, hi -
+          // lo <= kWinPairs - 2, so i + 1 stays inside).  Each step adds
+          // d[i] += w (1 - f) m, d[i + 1] += w f m of the four partners with
+          // LDS atomics (the same i, f, w, mask and sums as the gather-form
+          // adjoint); a column's slot is flushed into the data cube (global
+          // atomic adds of its nonzero samples) in the group after its last
+          // step and zeroed there for the column that takes the slot next
+          // (the double ring: that column's first step is a group later).
+          using V2 = float __attribute__((ext_vector_type(2)));
+          const int* const wid = reinterpret_cast<const int*>(smem + PL::kWid);
+          const int jsh = two ? 2 : 1;  // log2 of the jobs per column
+          auto job_at = [&](int oa, int nc, int jj) {
+            return StageJob{oa + (jj >> jsh), (jj >> (jsh - 1)) & 1, jj & (jsh - 1), jj < (nc << jsh)};
+          };
+          auto tab = [&](int side, int o) { return meta[side * PL::kCols + o]; };
+          // flush column jb.o of jb.side, ring row jb.rr: lane l holds the
+          // column's samples lo + 2l, lo + 2l + 1
+          auto flush = [&](const StageJob& jb) {
+            if (!jb.ok) return;
+            const int sl = PL::slot(jb.side, jb.o);
+            char* const cell = smem + sl + jb.rr * PL::kRow + lane * 8;
+            const V2 v = *reinterpret_cast<const V2*>(cell);
+            *reinterpret_cast<V2*>(cell) = V2{0.0f, 0.0f};
+            const int k = jb.side == 0 ? kbase + jb.o : kbase + (W - 1) - jb.o;
+            if (k < xd_lo || k >= xd_hi) return;  // outside the data: nothing scattered there counts
+            const int lo = (sl - tab(jb.side, jb.o)) / kB;
+            const int n = wid[jb.side * PL::kCols + jb.o];
+            float* const dst = a.adj_out +
+                               ((size_t)(jb.rr == 0 ? r0 : rm) * a.n_xd + (k - xd_lo)) * a.n_t + lo +
+                               2 * lane;
+            if (2 * lane < n && v.x != 0.0f) unsafeAtomicAdd(dst, v.x);
+            if (2 * lane + 1 < n && v.y != 0.0f) unsafeAtomicAdd(dst + 1, v.y);
+          };
+          constexpr int kJobs = (4 * G + W - 1) / W;  // flush jobs per wave per group, at most
+          using Tab = StageTab<G, 1>;
+          auto read_tab = [&](int dk0, Tab& t) {
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+              const int dk = dk0 + u;
+              t.r2[u] = r2tab[dk];
+              t.bR[u] = tab(0, w + dk);
+              t.bL[u] = tab(1, (W - 1 - w) + dk);
+            }
+          };
+          auto lds_add = [&](int addr, float v) {
+            __hip_atomic_fetch_add(reinterpret_cast<float*>(smem + addr), v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          };
+          // steps dk0 + u, u < n (FULL: all G); FIRST: dk = 0 has one column;
+          // MASK = false: every valid lane is inside its aperture
+          auto scatter = [&](auto nr_c, auto full_c, auto first_c, auto mask_c, int dk0, int n,
+                             const Tab& t) {
+            constexpr int NR = decltype(nr_c)::value;
+            constexpr bool FULL = decltype(full_c)::value;
+            constexpr bool FIRST = decltype(first_c)::value;
+            constexpr bool MASK = decltype(mask_c)::value;
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+              if (FULL || u < n) {
+                const bool one = FIRST && u == 0;
+                const float r2 = t.r2[u];
+                const float tau = sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq));
+                const float q = tau * a.inv_dt;
+                const int i = (int)q;
+                const bool on = (!MASK || r2 <= p.R2) && i < nt1;
+                const float f = __builtin_amdgcn_fractf(q);
+                const float wt = AMP ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+                const float wm = wt * p.mval;
+                const float c0 = wm - wm * f, c1 = wm * f;
+                int ar, al;
+                asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(ar) : "v"(i), "v"(t.bR[u]));
+                asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(al) : "v"(i), "v"(t.bL[u]));
+                // only lanes with a term add: the others would all hit one
+                // address (invalid lanes share the workgroup's lower corner,
+                // lanes past the trace end one sample) and serialize the atomics
+                // (all samples i first, then all i + 1: the lane above
+                // usually adds at this lane's i + 1, and atomics on one
+                // address in consecutive instructions stall the LDS)
+                if (on) {
+                  lds_add(ar, c0);
+                  if (!one) lds_add(al, c0);
+                  if constexpr (NR == 2) {
+                    lds_add(ar + PL::kRow, c0);
+                    if (!one) lds_add(al + PL::kRow, c0);
+                  }
+                  lds_add(ar + 4, c1);
+                  if (!one) lds_add(al + 4, c1);
+                  if constexpr (NR == 2) {
+                    lds_add(ar + PL::kRow + 4, c1);
+                    if (!one) lds_add(al + PL::kRow + 4, c1);
+                  }
+                }
+              }
+            }
+          };
+          int flushed = 0;  // columns [0, flushed) are flushed
+          auto run_group = [&](auto nr_c, auto full_c, auto first_c, auto mask_c, int dk0, Tab& t) {
+            if (dk0 >= G) {  // the previous group's G columns are complete
+#pragma unroll
+              for (int qq = 0; qq < kJobs; ++qq) flush(job_at(dk0 - G, G, w + qq * W));
+              flushed = dk0;
+            }
+            scatter(nr_c, full_c, first_c, mask_c, dk0, min(G, kmax_wg + 1 - dk0), t);
+            read_tab(dk0 + G, t);
+            __syncthreads();  // the group's sums are in; the zeroed slots are visible
+          };
+          auto run_row = [&](auto nr_c) {
+            using T = std::integral_constant<bool, true>;
+            using F = std::integral_constant<bool, false>;
+            Tab t;
+            read_tab(0, t);
+            if (kmax_wg + 1 >= G)
+              run_group(nr_c, T{}, T{}, T{}, 0, t);
+            else
+              run_group(nr_c, F{}, T{}, T{}, 0, t);
+            int dk0 = G;
+            for (; dk0 + G <= min(kmax_wg, kin) + 1; dk0 += G) run_group(nr_c, T{}, F{}, F{}, dk0, t);
+            for (; dk0 + G <= kmax_wg + 1; dk0 += G) run_group(nr_c, T{}, F{}, T{}, dk0, t);
+            if (dk0 <= kmax_wg) run_group(nr_c, F{}, F{}, T{}, dk0, t);
+            // the remaining columns [flushed, nO)
+            const int nc = nO - flushed;
+            for (int jj = w; jj < (nc << jsh); jj += W) flush(job_at(flushed, nc, jj));
+          };
+          if (two)
+            run_row(std::integral_constant<int, 2>{});
+          else
+            run_row(std::integral_constant<int, 1>{});
+          continue;
+        }
+      } else if (!fallback) {
         // ---- staging jobs: job j of a set of columns [oa, oa + nc) is
         // (column oa + j / (2 nr), side (j / nr) % 2, ring row j % nr), nr =
         // rows of the pair; wave w takes jobs w, w + W, ...  A job loads the
@@ -1167,21 +1320,23 @@ __device__ __forceinline__ void staged_tile(const MigArgs& a, int tile, int io, 
       }
       // gather path for this row
       if (R2w >= d2)
-        sweep_row<ST, false, kForward>(a, kj, p, R2w, acc, cap, cg,
+        sweep_row<ST, false, MODE>(a, kj, p, R2w, acc, cap, cg,
                                        hp ? base + (size_t)rp * row_bytes : zero_row, hp,
                                        hp ? rp : a.n_rows,
                                        hm ? base + (size_t)rm * row_bytes : zero_row, hm,
                                        hm ? rm : a.n_rows, d2);
     }
   }
-  if (in_out && active) {
-    float* orow = a.out + ((size_t)io * a.n_out_xl + j) * a.n_t;
-    if (!in_buf) {
-      if (!a.accumulate) orow[s] = 0.0f;  // migration_kernel.cpp:79-85
-    } else if (a.accumulate) {
-      if (valid) orow[s] += acc;
-    } else {
-      orow[s] = valid ? acc : 0.0f;  // :148-151, :375
+  if constexpr (MODE == kForward) {
+    if (in_out && active) {
+      float* orow = a.out + ((size_t)io * a.n_out_xl + j) * a.n_t;
+      if (!in_buf) {
+        if (!a.accumulate) orow[s] = 0.0f;  // migration_kernel.cpp:79-85
+      } else if (a.accumulate) {
+        if (valid) orow[s] += acc;
+      } else {
+        orow[s] = valid ? acc : 0.0f;  // :148-151, :375
+      }
     }
   }
 }
@@ -1199,6 +1354,20 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
   const int spatial = (rem % kXcds) * (per_tile_pad / kXcds) + rem / kXcds;
   if (spatial >= a.n_out_il * n_jgroups) return;  // whole workgroup
   staged_tile<ST, W, G, AMP>(a, tile, spatial / n_jgroups, (spatial % n_jgroups) * W, smem);
+}
+
+// The staged adjoint over the same grid as migrate_lds_kernel (FP32 data).
+template <int W, int G, bool AMP>
+__global__ __launch_bounds__(kWave * W) void demigrate_lds_kernel(MigArgs a, int n_tiles,
+                                                                  int n_jgroups, int per_tile_pad) {
+  __shared__ __attribute__((aligned(16))) char smem[LdsPlan<PSM_STORE_F32, W, G, kAdjoint>::kBytes];
+  const int g = blockIdx.x;
+  const int tile = n_tiles - 1 - g / per_tile_pad;
+  const int rem = g % per_tile_pad;
+  const int spatial = (rem % kXcds) * (per_tile_pad / kXcds) + rem / kXcds;
+  if (spatial >= a.n_out_il * n_jgroups) return;  // whole workgroup
+  staged_tile<PSM_STORE_F32, W, G, AMP, kAdjoint>(a, tile, spatial / n_jgroups,
+                                                 (spatial % n_jgroups) * W, smem);
 }
 
 // Persistent batch kernel: every workgroup takes work items (one
@@ -1475,6 +1644,24 @@ hipError_t launch_lds_wg(const MigArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// staged adjoint: 8 output crosslines per workgroup, 2 offsets per group
+hipError_t launch_adj_lds(const MigArgs& a, hipStream_t stream) {
+  constexpr int W = kAutoStageWaves, G = kAutoStageSteps;
+  const int n_tiles = (a.n_t + kWave - 1) / kWave;
+  const int n_jgroups = (a.n_out_xl + W - 1) / W;
+  const long long per_tile = (long long)a.n_out_il * n_jgroups;
+  const long long per_tile_pad = (per_tile + kXcds - 1) / kXcds * kXcds;
+  const long long blocks = per_tile_pad * n_tiles;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (a.amp)
+    hipLaunchKernelGGL((demigrate_lds_kernel<W, G, true>), dim3((unsigned)blocks), dim3(kWave * W),
+                       0, stream, a, n_tiles, n_jgroups, (int)per_tile_pad);
+  else
+    hipLaunchKernelGGL((demigrate_lds_kernel<W, G, false>), dim3((unsigned)blocks),
+                       dim3(kWave * W), 0, stream, a, n_tiles, n_jgroups, (int)per_tile_pad);
+  return hipGetLastError();
+}
+
 template <int ST>
 hipError_t launch_lds_st(const MigArgs& a, hipStream_t stream, int wpb, int steps) {
   if (wpb == 8)
@@ -1508,9 +1695,12 @@ hipError_t launch_migrate(const MigArgs& a0, int mode, hipStream_t stream, const
   const int wpb = auto_lds ? kAutoStageWaves : tune.waves_per_block;
   const int steps = auto_lds ? kAutoStageSteps : tune.stage_steps;
   const bool lds = auto_lds || (tune.staging > 0 && lds_ok && (wpb == 4 || wpb == 8));
+  // staged adjoint: contiguous rows, unless staging is switched off
+  const bool adj_lds = mode == kAdjoint && a0.row_pos == nullptr && tune.staging != 0 &&
+                       (size_t)(a0.n_xd + 1) * a0.n_t * 4 <= kLdsMaxRowBytes;
   // automatic passes: 8 row offsets per launch for FP32 gathers, one launch otherwise
   const int rpp = tune.rows_per_pass >= 0 ? tune.rows_per_pass
-                                          : ((f32_traces && !lds) ? kAutoRowsF32 : 0);
+                                          : ((f32_traces && !lds && !adj_lds) ? kAutoRowsF32 : 0);
   const int R = (mode == kCount || rpp <= 0) ? (row_extent + 1) : rpp;
   for (int lo = 0, pass = 0; lo <= row_extent; lo += R, ++pass) {
     a.d_lo = lo;
@@ -1519,7 +1709,9 @@ hipError_t launch_migrate(const MigArgs& a0, int mode, hipStream_t stream, const
     hipError_t e;
     switch (mode) {
       case kCount: e = launch_st<PSM_STORE_F32, kCount>(a, stream, wpb); break;
-      case kAdjoint: e = launch_st<PSM_STORE_F32, kAdjoint>(a, stream, wpb); break;
+      case kAdjoint:
+        e = adj_lds ? launch_adj_lds(a, stream) : launch_st<PSM_STORE_F32, kAdjoint>(a, stream, wpb);
+        break;
       case kTangent: e = launch_st<PSM_STORE_F32, kTangent>(a, stream, wpb); break;
       default:
         if (lds) {

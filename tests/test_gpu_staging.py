@@ -109,6 +109,31 @@ def test_staged_matches_reference_and_gather(ctx, name):
             assert e_g <= TOL_FORMS, (wpb, steps, e_g)
 
 
+@pytest.mark.parametrize("name", CASE_NAMES)
+def test_staged_adjoint_matches_gather_adjoint(ctx, name):
+    """The staged adjoint (the workgroup's image samples scattered into LDS
+    windows, each window flushed once into the data cube) against the
+    gather-form adjoint (every pair scattered with global atomics): the same
+    terms summed in another order.  The rows whose windows or offsets do not
+    fit (coarse_slow, fine_bins) and sticky workgroups take the gather-form
+    adjoint inside the staged kernel."""
+    data, vel, prm, _ = _case(name)
+    info = P.survey_plan(prm, vel)
+    rng = np.random.default_rng(11)
+    image = rng.standard_normal((info.n_out_il, info.n_out_xl, info.n_t)).astype(np.float32)
+    print(f"adjoint case {name} image {image.shape}", flush=True)
+    ctx.set_staging(False)
+    try:
+        gather = P.demigrate_volume(image, vel, prm, ctx=ctx)
+    finally:
+        ctx.set_staging(None, 2)
+    staged = P.demigrate_volume(image, vel, prm, ctx=ctx)
+    assert np.isfinite(staged).all()
+    e = rel_l2(staged, gather)
+    print(f"  staged vs gather adjoint {e:.2e}", flush=True)
+    assert e <= TOL_FORMS, e
+
+
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 @pytest.mark.parametrize("name", ["padded_lateral", "coarse_slow"])
 def test_staged_half_storage_matches_gather(ctx, name, storage):
