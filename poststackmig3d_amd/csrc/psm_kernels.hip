@@ -712,8 +712,8 @@ constexpr size_t kLdsMaxRowBytes = (size_t)1 << 30;
 // kAdjoint scatters the image samples into accumulator windows that are
 // flushed into the data cube (ST = F32).
 template <int ST, int W, int G, bool AMP, int MODE = kForward>
-__device__ __forceinline__ void staged_tile(const MigArgs& a, int tile, int io, int j0,
-                                            char* smem) {
+__device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int tile, int io,
+                                            int j0, char* smem) {
   using S = Store<ST>;
   using Raw = typename S::Raw;
   using PL = LdsPlan<ST, W, G, MODE>;
@@ -726,7 +726,6 @@ __device__ __forceinline__ void staged_tile(const MigArgs& a, int tile, int io, 
   const bool active = s < a.n_t;
   const bool in_out = j < a.n_out_xl;
   const bool in_buf = kj >= 0 && kj < a.K;
-  const Task tk = const_load<Task>(a.tasks + io);  // scalar, as the row indices derive from it
 
   LaneSetup ls{{0.0f, 0.0f, -1.0f, 1.0f, 0, 0, 0.0f}, -1, false, -1.0f};
   if (in_out && in_buf) ls = lane_setup<MODE>(a, tk, io, j, kj, s, active);
@@ -1353,7 +1352,8 @@ __global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n
   const int rem = g % per_tile_pad;
   const int spatial = (rem % kXcds) * (per_tile_pad / kXcds) + rem / kXcds;
   if (spatial >= a.n_out_il * n_jgroups) return;  // whole workgroup
-  staged_tile<ST, W, G, AMP>(a, tile, spatial / n_jgroups, (spatial % n_jgroups) * W, smem);
+  const int io = spatial / n_jgroups;
+  staged_tile<ST, W, G, AMP>(a, a.tasks[io], tile, io, (spatial % n_jgroups) * W, smem);
 }
 
 // The staged adjoint over the same grid as migrate_lds_kernel (FP32 data).
@@ -1366,7 +1366,8 @@ __global__ __launch_bounds__(kWave * W) void demigrate_lds_kernel(MigArgs a, int
   const int rem = g % per_tile_pad;
   const int spatial = (rem % kXcds) * (per_tile_pad / kXcds) + rem / kXcds;
   if (spatial >= a.n_out_il * n_jgroups) return;  // whole workgroup
-  staged_tile<PSM_STORE_F32, W, G, AMP, kAdjoint>(a, tile, spatial / n_jgroups,
+  const int io = spatial / n_jgroups;
+  staged_tile<PSM_STORE_F32, W, G, AMP, kAdjoint>(a, a.tasks[io], tile, io,
                                                  (spatial % n_jgroups) * W, smem);
 }
 
@@ -1437,10 +1438,12 @@ void batch_lds_kernel(const MigArgs* __restrict__ items, const BatchSeg* __restr
     const int io = spatial / sg.n_jgroups;
     const int j0 = (spatial % sg.n_jgroups) * W;
     if (io < a.n_out_il) {
+      // (the task through scalar loads too: the row indices derive from it)
+      const Task tk = const_load<Task>(a.tasks + io);
       if (a.amp)
-        staged_tile<ST, W, G, true>(a, sg.tile, io, j0, smem);
+        staged_tile<ST, W, G, true>(a, tk, sg.tile, io, j0, smem);
       else
-        staged_tile<ST, W, G, false>(a, sg.tile, io, j0, smem);
+        staged_tile<ST, W, G, false>(a, tk, sg.tile, io, j0, smem);
     }
   }
 }
