@@ -691,7 +691,8 @@ struct LdsPlan {
   static constexpr int kPar = kR2 + kCols * 4;           // float par[5][W]
   static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
   static constexpr int kWid = kFlag + W * 4;             // adjoint: int wid[2][kCols]
-  static constexpr int kBytes = kWid + (kAdj ? 2 * kCols * 4 : 0);
+  static constexpr int kClaim = kWid + (kAdj ? 2 * kCols * 4 : 0);  // adjoint: int claim[W][kWinPairs]
+  static constexpr int kBytes = kClaim + (kAdj ? W * kWinPairs * 4 : 0);
   // waves per SIMD the LDS allows (three 8-wave workgroups per CU with the
   // double ring), else no register target
   static constexpr int kWavesPerEu = (W == 8 && !kSingle && 3 * (kBytes + 16) <= 160 * 1024) ? 6 : 1;
@@ -941,9 +942,20 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
               t.bL[u] = tab(1, (W - 1 - w) + dk);
             }
           };
-          auto lds_add = [&](int addr, float v) {
-            __hip_atomic_fetch_add(reinterpret_cast<float*>(smem + addr), v, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          // Plain read-add-write of the two samples i, i + 1 of a window
+          // (LDS FP32 atomics run at about one lane per cycle).  Between
+          // barriers every window belongs to one wave (one step per group:
+          // at a step the waves touch distinct columns), so only lanes of
+          // one wave with the same i collide: each round, the lanes still
+          // pending claim their i in the wave's claim row (one writer per
+          // address wins), the winners add, the others retry.  i mod
+          // kWinPairs identifies i: a wave's i at one step lie in one window.
+          static_assert(G == 1, "one step per group: windows private to one wave between barriers");
+          using LdsInt = volatile __attribute__((address_space(3))) int;  // LDS, not flat
+          LdsInt* const claim = (LdsInt*)(smem + PL::kClaim) + w * kWinPairs;
+          auto rmw = [&](int addr, float c) {
+            float* const cell = reinterpret_cast<float*>(smem + addr);
+            *cell = *cell + c;
           };
           // steps dk0 + u, u < n (FULL: all G); FIRST: dk = 0 has one column;
           // MASK = false: every valid lane is inside its aperture
@@ -969,25 +981,34 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                 int ar, al;
                 asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(ar) : "v"(i), "v"(t.bR[u]));
                 asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(al) : "v"(i), "v"(t.bL[u]));
-                // only lanes with a term add: the others would all hit one
-                // address (invalid lanes share the workgroup's lower corner,
-                // lanes past the trace end one sample) and serialize the atomics
-                // (all samples i first, then all i + 1: the lane above
-                // usually adds at this lane's i + 1, and atomics on one
-                // address in consecutive instructions stall the LDS)
-                if (on) {
-                  lds_add(ar, c0);
-                  if (!one) lds_add(al, c0);
-                  if constexpr (NR == 2) {
-                    lds_add(ar + PL::kRow, c0);
-                    if (!one) lds_add(al + PL::kRow, c0);
+                // only lanes with a term take part (i of the others may lie
+                // outside the window)
+                const int ci = i & (kWinPairs - 1);
+                bool pending = on;
+                while (__any(pending)) {
+                  asm volatile("" ::: "memory");  // the previous round's adds come first
+                  if (pending) claim[ci] = lane;
+                  const bool win = pending && claim[ci] == lane;
+                  // samples i first, then i + 1 (the winner at i + 1 adds there
+                  // too): within each phase the winners' cells are distinct
+                  if (win) {
+                    rmw(ar, c0);
+                    if (!one) rmw(al, c0);
+                    if constexpr (NR == 2) {
+                      rmw(ar + PL::kRow, c0);
+                      if (!one) rmw(al + PL::kRow, c0);
+                    }
                   }
-                  lds_add(ar + 4, c1);
-                  if (!one) lds_add(al + 4, c1);
-                  if constexpr (NR == 2) {
-                    lds_add(ar + PL::kRow + 4, c1);
-                    if (!one) lds_add(al + PL::kRow + 4, c1);
+                  asm volatile("" ::: "memory");
+                  if (win) {
+                    rmw(ar + 4, c1);
+                    if (!one) rmw(al + 4, c1);
+                    if constexpr (NR == 2) {
+                      rmw(ar + PL::kRow + 4, c1);
+                      if (!one) rmw(al + PL::kRow + 4, c1);
+                    }
                   }
+                  pending = pending && !win;
                 }
               }
             }
@@ -1647,9 +1668,9 @@ hipError_t launch_lds_wg(const MigArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-// staged adjoint: 8 output crosslines per workgroup, 2 offsets per group
+// staged adjoint: 8 output crosslines per workgroup, one offset per group
 hipError_t launch_adj_lds(const MigArgs& a, hipStream_t stream) {
-  constexpr int W = kAutoStageWaves, G = kAutoStageSteps;
+  constexpr int W = kAutoStageWaves, G = 1;
   const int n_tiles = (a.n_t + kWave - 1) / kWave;
   const int n_jgroups = (a.n_out_xl + W - 1) / W;
   const long long per_tile = (long long)a.n_out_il * n_jgroups;
