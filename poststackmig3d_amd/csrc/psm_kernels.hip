@@ -953,9 +953,25 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           static_assert(G == 1, "one step per group: windows private to one wave between barriers");
           using LdsInt = volatile __attribute__((address_space(3))) int;  // LDS, not flat
           LdsInt* const claim = (LdsInt*)(smem + PL::kClaim) + w * kWinPairs;
-          auto rmw = [&](int addr, float c) {
-            float* const cell = reinterpret_cast<float*>(smem + addr);
-            *cell = *cell + c;
+          // adds c at byte offset `off` of the partners' cells (addresses
+          // ar, al in ring row 0, + kRow in ring row 1): all reads, then all
+          // writes (the partners' windows are distinct slots)
+          auto rmw = [&](auto nr_c, bool one, int ar, int al, int off, float c) {
+            constexpr int NR = decltype(nr_c)::value;
+            float* const p0 = reinterpret_cast<float*>(smem + ar + off);
+            float* const p1 = reinterpret_cast<float*>(smem + al + off);
+            float* const p2 = reinterpret_cast<float*>(smem + ar + PL::kRow + off);
+            float* const p3 = reinterpret_cast<float*>(smem + al + PL::kRow + off);
+            const float x0 = *p0;
+            const float x1 = one ? 0.0f : *p1;
+            const float x2 = NR == 2 ? *p2 : 0.0f;
+            const float x3 = (NR == 2 && !one) ? *p3 : 0.0f;
+            *p0 = x0 + c;
+            if (!one) *p1 = x1 + c;
+            if constexpr (NR == 2) {
+              *p2 = x2 + c;
+              if (!one) *p3 = x3 + c;
+            }
           };
           // steps dk0 + u, u < n (FULL: all G); FIRST: dk = 0 has one column;
           // MASK = false: every valid lane is inside its aperture
@@ -991,23 +1007,9 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                   const bool win = pending && claim[ci] == lane;
                   // samples i first, then i + 1 (the winner at i + 1 adds there
                   // too): within each phase the winners' cells are distinct
-                  if (win) {
-                    rmw(ar, c0);
-                    if (!one) rmw(al, c0);
-                    if constexpr (NR == 2) {
-                      rmw(ar + PL::kRow, c0);
-                      if (!one) rmw(al + PL::kRow, c0);
-                    }
-                  }
+                  if (win) rmw(nr_c, one, ar, al, 0, c0);
                   asm volatile("" ::: "memory");
-                  if (win) {
-                    rmw(ar + 4, c1);
-                    if (!one) rmw(al + 4, c1);
-                    if constexpr (NR == 2) {
-                      rmw(ar + PL::kRow + 4, c1);
-                      if (!one) rmw(al + PL::kRow + 4, c1);
-                    }
-                  }
+                  if (win) rmw(nr_c, one, ar, al, 4, c1);
                   pending = pending && !win;
                 }
               }
