@@ -900,8 +900,8 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           // ---- adjoint: a column's slot accumulates the data samples
           // lo .. lo + kWinPairs - 1 of its trace (lo as in the table, hi -
           // lo <= kWinPairs - 2, so i + 1 stays inside).  Each step adds
-          // d[i] += w (1 - f) m, d[i + 1] += w f m of the four partners with
-          // LDS atomics (the same i, f, w, mask and sums as the gather-form
+          // d[i] += w (1 - f) m, d[i + 1] += w f m of the four partners into
+          // the slots (the same i, f, w, mask and terms as the gather-form
           // adjoint); a column's slot is flushed into the data cube (global
           // atomic adds of its nonzero samples) in the group after its last
           // step and zeroed there for the column that takes the slot next
