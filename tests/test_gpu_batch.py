@@ -42,7 +42,9 @@ def _reference(items, storage, wpb, staged):
     try:
         ctx.set_storage(storage)
         if staged:
-            ctx.set_staging(None)  # the automatic form: staged, 8 crosslines, one launch
+            # the batch's staged form: 8 crosslines, 2 offsets per group, one launch
+            ctx.set_tuning(8, 0)
+            ctx.set_staging(True, 2)
         else:
             ctx.set_tuning(wpb, 0)
             ctx.set_staging(False)
@@ -55,12 +57,13 @@ def _reference(items, storage, wpb, staged):
         ctx.close()
 
 
-@pytest.mark.parametrize("storage,staging", [("f32", None), ("f16", None), ("f32", False)])
+@pytest.mark.parametrize("storage,staging",
+                         [("f32", None), ("f16", None), ("f32", False), ("f16", True)])
 def test_batch_matches_single_volumes(storage, staging):
     import torch
     from poststackmig3d_amd.batch import Batch
     items = _items(torch, storage)
-    staged = storage == "f32" and staging is None
+    staged = staging is True or (storage == "f32" and staging is None)
     refs = _reference(items, storage, 4, staged)
     bctx = Context(0)
     bctx.set_staging(staging)
