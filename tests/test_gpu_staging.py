@@ -52,6 +52,14 @@ def _case(name):
         data, _ = synth.point_diffractor(*n, dy=0.5, noise=0.1, seed=6)
         vel = np.full(n, 2000.0, np.float32)
         geo = dict(inline_step=25.0, crossline_step=0.5, angle_aperture=60.0)
+    elif name == "mid_bins":
+        # 100 m crossline bins at 2000 m/s, 60 deg: the windows of the first
+        # columns of a row fit a slot while columns further out exceed it, so
+        # the row's fallback vote must see the columns of every lane
+        n = (9, 60, 600)
+        data, _ = synth.point_diffractor(*n, dy=100.0, v=2000.0, noise=0.2, seed=7)
+        vel = np.full(n, 2000.0, np.float32)
+        geo = dict(inline_step=25.0, crossline_step=100.0, angle_aperture=60.0)
     elif name == "short_traces":
         # 40 samples: every window reaches the trace end, and the staging
         # loads stop at it (the lane holding the last pair reads into the next
@@ -69,7 +77,8 @@ def _case(name):
     return data, vel, prm, ref_args
 
 
-CASE_NAMES = ["padded_lateral", "point_unpadded", "coarse_slow", "fine_bins", "short_traces"]
+CASE_NAMES = ["padded_lateral", "point_unpadded", "coarse_slow", "mid_bins", "fine_bins",
+              "short_traces"]
 
 
 @pytest.fixture(scope="module")
