@@ -37,8 +37,10 @@
 //
 // Arithmetic follows the reference's compiled order: r^2 = fma(dk dy, dk dy,
 // dx^2) and tau^2 = fma(4/v^2, r^2, t0^2) are single-rounding FMAs (the
-// -ffast-math build contracts them), tau is the correctly rounded square root,
-// q = tau * (1/dt) with 1/dt rounded once.  The file is compiled with
+// -ffast-math build contracts them), q = tau * (1/dt) with 1/dt rounded once.
+// tau is the correctly rounded square root (the summing modes take it and
+// the weight from one reciprocal square root and a Newton step, the pair
+// count from a rounded root and a neighbour test).  The file is compiled with
 // -ffp-contract=off so that these FMAs are exactly the ones written here.
 #include <hip/hip_runtime.h>
 
@@ -67,18 +69,6 @@ __device__ __forceinline__ float sqrt_rn(float x) {
   float r = (r_dn <= 0.0f) ? s_dn : s;
   r = (r_up > 0.0f) ? s_up : r;
   return r;
-}
-
-// Square root that is correctly rounded except in rare cases: v_sqrt_f32 on
-// gfx950 returns the correctly rounded root or the float just below it
-// (measured over 6.7e7 arguments in [1e-6, 1e2]: 15.4 % one below, 0.006 %
-// one above, none further off), so testing the upper neighbour alone gives
-// the correctly rounded root for all but ~6e-5 of the arguments, which come
-// out one ulp high.
-__device__ __forceinline__ float sqrt_up(float x) {
-  const float s = __builtin_amdgcn_sqrtf(x);
-  const float s_up = __int_as_float(__float_as_int(s) + 1);
-  return (__builtin_fmaf(-s_up, s, x) > 0.0f) ? s_up : s;
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -183,9 +173,33 @@ struct Store<PSM_STORE_BF16> {
 // for the crossline `src` whose velocity the lane uses.
 struct LaneParams {
   float t0sq, a, R2, v2;
+  float iv2;  // 1 / (v^2 + 1e-10) = a / 4
   int wlo, whi;
   float mval;  // adjoint: the image sample this lane scatters
 };
+
+// Travel time and amplitude weight of one pair from one reciprocal square
+// root (migration_kernel.cpp:212, 270): with y = t0^2 + a r^2 and
+// rs = rsq(y) (v_rsq_f32, within 1 ulp), s = y rs is sqrt(y) to about
+// 1.5 ulp, and one Newton step tau = s + (y - s^2) rs / 2 (the residual
+// from an FMA) gives the correctly rounded root but for arguments within a
+// tiny fraction of an ulp of a rounding midpoint -- the reference rounds tau
+// correctly, and the bound i < n_t - 1 makes terms at the trace end depend
+// on the last bit of tau.  The weight 1 / (tau v^2 + 1e-10) is taken as
+// rs / (v^2 + 1e-10) (the two differ by a relative 1e-10 (tau - 1) /
+// (tau v^2), nothing at seismic velocities).  The forward, adjoint and
+// tangent all use this one form, so the adjoint is the exact transpose of
+// the forward.
+struct TauW {
+  float tau, w;
+};
+__device__ __forceinline__ TauW tau_weight(const LaneParams& p, float r2, bool amp) {
+  const float y = __builtin_fmaf(p.a, r2, p.t0sq);
+  const float rs = __builtin_amdgcn_rsqf(y);
+  const float s = y * rs;
+  const float e = __builtin_fmaf(-s, s, y);
+  return TauW{__builtin_fmaf(e, 0.5f * rs, s), amp ? rs * p.iv2 : 1.0f};
+}
 
 // Sums one aperture row pair of one (output trace, t0 tile): rows +db and
 // -db of the output inline (rowp / rowm, present when hp / hm; ip / im their
@@ -275,8 +289,8 @@ __device__ __forceinline__ void sweep_row(const MigArgs& a, int kj, const LanePa
           col[c] = ok ? col[c] : a.n_xd;
           off[c] = col[c] * tb;
         }
-        const float tau = sqrt_up(__builtin_fmaf(p.a, r2, p.t0sq));
-        const float q = tau * a.inv_dt;
+        const TauW tw = tau_weight(p, r2, a.amp);
+        const float q = tw.tau * a.inv_dt;
         const int voff = min((int)q, nt1) * S::kBytes;
 #pragma unroll
         for (int c = 0; c < NC; ++c) g.x[c] = S::load(rs0, voff, off[c]);
@@ -293,7 +307,7 @@ __device__ __forceinline__ void sweep_row(const MigArgs& a, int kj, const LanePa
           }
         }
         g.f = __builtin_amdgcn_fractf(q);
-        g.w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+        g.w = tw.w;
         g.m = r2 <= p.R2;
       };
       // a masked step adds 0 * w: the gathered sum is selected away before
@@ -370,7 +384,10 @@ __device__ __forceinline__ void sweep_row(const MigArgs& a, int kj, const LanePa
       const bool has_km = dk > 0 && km >= xd_lo && km < xd_hi;
       if (!has_kp && !has_km) continue;
       const bool in_mask = r2 <= p.R2;
-      const float tau = sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq));
+      // (the count evaluates i from the correctly rounded root)
+      const TauW tw = COUNT ? TauW{sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq)), 1.0f}
+                            : tau_weight(p, r2, a.amp);
+      const float tau = tw.tau;
       const float q = tau * a.inv_dt;
       const int i = (int)q;
       bool okp = true, okm = true;
@@ -388,8 +405,7 @@ __device__ __forceinline__ void sweep_row(const MigArgs& a, int kj, const LanePa
         // the interpolation fma(f, x[i+1] - x[i], x[i]))
         if (in_mask && i < nt1) {
           const float f = __builtin_amdgcn_fractf(q);
-          const float w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
-          const float wm = w * p.mval;
+          const float wm = tw.w * p.mval;
           const float c0 = wm - wm * f, c1 = wm * f;
           auto scatter = [&](bool row_ok, int row, bool k_ok, int k) {
             if (row_ok && k_ok) {
@@ -428,7 +444,7 @@ __device__ __forceinline__ void sweep_row(const MigArgs& a, int kj, const LanePa
         l3 = okm ? l3 : 0.0f;
         float sum = (l0 + l1) + (l2 + l3);
         sum = in_mask ? sum : 0.0f;
-        const float w = a.amp ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+        const float w = tw.w;
         if constexpr (MODE == kTangent) {
           // d/d(alpha) of w * lerp(x, tau/dt) with v -> alpha v and the mask
           // held fixed: dtau = -(a r^2) / (alpha tau), dq = dtau / dt,
@@ -526,7 +542,7 @@ __device__ __forceinline__ LaneSetup lane_setup(const MigArgs& a, const Task& tk
       }
     }
   }
-  LaneParams p{0.0f, 0.0f, -1.0f, 1.0f, 0, 0, 0.0f};
+  LaneParams p{1.0f, 0.0f, -1.0f, 1.0f, 0.0f, 0, 0, 0.0f};
   bool needs_window = false;
   if (src >= 0) {
     // velocity scale: travel time and weight use alpha v, the aperture mask
@@ -536,6 +552,7 @@ __device__ __forceinline__ LaneSetup lane_setup(const MigArgs& a, const Task& tk
     p.t0sq = t0 * t0;
     const float inv_v2 = 1.0f / (vt * vt + 1e-10f);
     p.a = 4.0f * inv_v2;
+    p.iv2 = inv_v2;
     const float vth = vm * t0 * 0.5f;
     p.R2 = vth * vth * a.tan2;
     const int w = (int)(sqrt_rn(p.R2) * a.inv_dy) + 1;
@@ -728,7 +745,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
   const bool in_out = j < a.n_out_xl;
   const bool in_buf = kj >= 0 && kj < a.K;
 
-  LaneSetup ls{{0.0f, 0.0f, -1.0f, 1.0f, 0, 0, 0.0f}, -1, false, -1.0f};
+  LaneSetup ls{{1.0f, 0.0f, -1.0f, 1.0f, 0.0f, 0, 0, 0.0f}, -1, false, -1.0f};
   if (in_out && in_buf) ls = lane_setup<MODE>(a, tk, io, j, kj, s, active);
   const bool valid = ls.src >= 0;
   const float R2w = uniform_f(wave_max(ls.L));
@@ -980,7 +997,6 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           // MASK = false: every valid lane is inside its aperture
           auto scatter = [&](auto nr_c, auto full_c, auto first_c, auto mask_c, int dk0, int n,
                              const Tab& t) {
-            constexpr int NR = decltype(nr_c)::value;
             constexpr bool FULL = decltype(full_c)::value;
             constexpr bool FIRST = decltype(first_c)::value;
             constexpr bool MASK = decltype(mask_c)::value;
@@ -989,13 +1005,12 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
               if (FULL || u < n) {
                 const bool one = FIRST && u == 0;
                 const float r2 = t.r2[u];
-                const float tau = sqrt_rn(__builtin_fmaf(p.a, r2, p.t0sq));
-                const float q = tau * a.inv_dt;
+                const TauW tw = tau_weight(p, r2, AMP);
+                const float q = tw.tau * a.inv_dt;
                 const int i = (int)q;
                 const bool on = (!MASK || r2 <= p.R2) && i < nt1;
                 const float f = __builtin_amdgcn_fractf(q);
-                const float wt = AMP ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
-                const float wm = wt * p.mval;
+                const float wm = tw.w * p.mval;
                 const float c0 = wm - wm * f, c1 = wm * f;
                 int ar, al;
                 asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(ar) : "v"(i), "v"(t.bR[u]));
@@ -1186,8 +1201,8 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
               const int dk = dk0 + u;
               const bool one = FIRST && u == 0;  // dk = 0: one column
               const float r2 = t.r2[u];
-              const float tau = sqrt_up(__builtin_fmaf(p.a, r2, p.t0sq));
-              const float q = tau * a.inv_dt;
+              const TauW tw = tau_weight(p, r2, AMP);
+              const float q = tw.tau * a.inv_dt;
               // both column addresses as (i << log2 kB) + base, one
               // instruction each (the compiler shares the shift otherwise)
               const int i = min((int)q, nt1);
@@ -1214,7 +1229,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                 }
               }
               st[u].f = __builtin_amdgcn_fractf(q);
-              st[u].wt = AMP ? __builtin_amdgcn_rcpf(__builtin_fmaf(tau, p.v2, 1e-10f)) : 1.0f;
+              st[u].wt = tw.w;
               if constexpr (MASK) st[u].m = r2 <= p.R2;
             }
           }
@@ -1370,8 +1385,9 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
 // workgroups: latest tiles first, consecutive spatial indices of a tile on one
 // XCD (as migrate_kernel).
 template <int ST, int W, int G, bool AMP>
-__global__ __launch_bounds__(kWave * W) void migrate_lds_kernel(MigArgs a, int n_tiles,
-                                                                int n_jgroups, int per_tile_pad) {
+__global__ __launch_bounds__(kWave * W)
+__attribute__((amdgpu_waves_per_eu(LdsPlan<ST, W, G>::kWavesPerEu)))
+void migrate_lds_kernel(MigArgs a, int n_tiles, int n_jgroups, int per_tile_pad) {
   __shared__ __attribute__((aligned(16))) char smem[LdsPlan<ST, W, G>::kBytes];
   const int g = blockIdx.x;
   const int tile = n_tiles - 1 - g / per_tile_pad;
