@@ -293,8 +293,8 @@ def main():
         return 0
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU only (the migration shards by output "
-              "inline ranges with no exchange; multi-GPU runs are not part of it).",
+        print("bench.py: measures one GPU only (the migration would shard by output "
+              "inline ranges with no exchange; multi-GPU runs are not implemented).",
               file=sys.stderr)
         return 2
 
