@@ -172,7 +172,7 @@ struct Store<PSM_STORE_BF16> {
 // Per-lane image-point parameters (migration_kernel.cpp:115-144), evaluated
 // for the crossline `src` whose velocity the lane uses.
 struct LaneParams {
-  float t0sq, a, R2, v2;
+  float t0sq, a, R2;
   float iv2;  // 1 / (v^2 + 1e-10) = a / 4
   int wlo, whi;
   float mval;  // adjoint: the image sample this lane scatters
@@ -542,7 +542,9 @@ __device__ __forceinline__ LaneSetup lane_setup(const MigArgs& a, const Task& tk
       }
     }
   }
-  LaneParams p{1.0f, 0.0f, -1.0f, 1.0f, 0.0f, 0, 0, 0.0f};
+  // (a lane without a sample keeps t0^2 = 1, so its reciprocal square root
+  // stays finite; its terms are masked or discarded)
+  LaneParams p{1.0f, 0.0f, -1.0f, 0.0f, 0, 0, 0.0f};
   bool needs_window = false;
   if (src >= 0) {
     // velocity scale: travel time and weight use alpha v, the aperture mask
@@ -558,7 +560,6 @@ __device__ __forceinline__ LaneSetup lane_setup(const MigArgs& a, const Task& tk
     const int w = (int)(sqrt_rn(p.R2) * a.inv_dy) + 1;
     p.wlo = max(0, src - w);
     p.whi = min(a.K, src + w + 1);
-    p.v2 = vt * vt;
     if constexpr (MODE == kAdjoint) p.mval = a.image[((size_t)io * a.n_out_xl + j) * a.n_t + s];
     if (src != kj) {
       // crosslines this lane could reach around kj, clipped to the data
@@ -745,7 +746,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
   const bool in_out = j < a.n_out_xl;
   const bool in_buf = kj >= 0 && kj < a.K;
 
-  LaneSetup ls{{1.0f, 0.0f, -1.0f, 1.0f, 0.0f, 0, 0, 0.0f}, -1, false, -1.0f};
+  LaneSetup ls{{1.0f, 0.0f, -1.0f, 0.0f, 0, 0, 0.0f}, -1, false, -1.0f};
   if (in_out && in_buf) ls = lane_setup<MODE>(a, tk, io, j, kj, s, active);
   const bool valid = ls.src >= 0;
   const float R2w = uniform_f(wave_max(ls.L));
