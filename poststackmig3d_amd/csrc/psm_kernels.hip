@@ -1301,7 +1301,12 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // group earlier (vs) and make them visible.
         auto run_group = [&](auto nr_c, auto full_c, auto first_c, auto mask_c, int dk0, Tab& t,
                              Jobs& vs, Dsts& ds, Jobs& vi, Dsts& di) {
+          // (wave priority: the loads go out first and the group's tail --
+          // stores and barrier -- is not held up behind other waves'
+          // sums; 0.75 % on the production survey)
+          __builtin_amdgcn_s_setprio(1);
           issue(t.mj, vi, di);
+          __builtin_amdgcn_s_setprio(0);
           compute(nr_c, full_c, first_c, mask_c, dk0, min(G, kmax_wg + 1 - dk0), t);
           read_tab(dk0 + G, t);
           // The stores wait on the loads, so the group's sum must be complete
@@ -1309,12 +1314,14 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           // no store is conditional (absent jobs write the spare slot), so no
           // summation is sunk past a store.
           asm volatile("" ::"v"(acc));
+          __builtin_amdgcn_s_setprio(2);
           __builtin_amdgcn_sched_barrier(0);
           if constexpr (PL::kSingle) __syncthreads();  // this group's reads are done
 #pragma unroll
           for (int q = 0; q < kJobs; ++q)
             if (lb < PL::kSlot) *reinterpret_cast<V4*>(smem + ds[q] + lb) = vs[q];
           __syncthreads();  // the next group's windows are visible
+          __builtin_amdgcn_s_setprio(0);
         };
         auto run_row = [&](auto nr_c) {
           using T = std::integral_constant<bool, true>;
