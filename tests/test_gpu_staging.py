@@ -144,7 +144,7 @@ def test_staged_adjoint_matches_gather_adjoint(ctx, name):
 
 
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
-@pytest.mark.parametrize("name", ["padded_lateral", "coarse_slow"])
+@pytest.mark.parametrize("name", ["padded_lateral", "coarse_slow", "mid_bins"])
 def test_staged_half_storage_matches_gather(ctx, name, storage):
     data, vel, prm, _ = _case(name)
     gather = _migrate(ctx, data, vel, prm, staging=False, storage=storage)
