@@ -710,7 +710,8 @@ struct LdsPlan {
   static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
   static constexpr int kWid = kFlag + W * 4;             // adjoint: int wid[2][kCols]
   static constexpr int kClaim = kWid + (kAdj ? 2 * kCols * 4 : 0);  // adjoint: int claim[W][kWinPairs]
-  static constexpr int kBytes = kClaim + (kAdj ? W * kWinPairs * 4 : 0);
+  static constexpr int kLP = kClaim + (kAdj ? W * kWinPairs * 4 : 0);  // adjoint: float lp[5][W][kWave]
+  static constexpr int kBytes = kLP + (kAdj ? 5 * W * 64 * 4 : 0);
   // waves per SIMD the LDS allows (three 8-wave workgroups per CU with the
   // double ring), else no register target
   static constexpr int kWavesPerEu = (W == 8 && !kSingle && 3 * (kBytes + 16) <= 160 * 1024) ? 6 : 1;
@@ -813,6 +814,27 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
     if (!valid) {
       p.t0sq = gTlo2;
       p.a = gAlo;
+    }
+    // adjoint: every wave holds the parameters of all W output crosslines
+    // (its lanes' samples at each of them), published once through LDS
+    LaneParams pw[PL::kAdj ? W : 1];
+    if constexpr (PL::kAdj) {
+      float* const lp = reinterpret_cast<float*>(smem + PL::kLP);
+      lp[(0 * W + w) * kWave + lane] = p.t0sq;
+      lp[(1 * W + w) * kWave + lane] = p.a;
+      lp[(2 * W + w) * kWave + lane] = p.R2;
+      lp[(3 * W + w) * kWave + lane] = p.iv2;
+      lp[(4 * W + w) * kWave + lane] = p.mval;
+      __syncthreads();
+#pragma unroll
+      for (int v = 0; v < W; ++v) {
+        pw[v] = p;
+        pw[v].t0sq = lp[(0 * W + v) * kWave + lane];
+        pw[v].a = lp[(1 * W + v) * kWave + lane];
+        pw[v].R2 = lp[(2 * W + v) * kWave + lane];
+        pw[v].iv2 = lp[(3 * W + v) * kWave + lane];
+        pw[v].mval = lp[(4 * W + v) * kWave + lane];
+      }
     }
 
     int dmax = (int)(__builtin_sqrtf(R2wg) / a.dx) + 2;
@@ -918,104 +940,53 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
       }
       if constexpr (PL::kAdj) {
         if (!fallback) {
-          // ---- adjoint: a column's slot accumulates the data samples
-          // lo .. lo + kWinPairs - 1 of its trace (lo as in the table, hi -
-          // lo <= kWinPairs - 2, so i + 1 stays inside).  Each step adds
-          // d[i] += w (1 - f) m, d[i + 1] += w f m of the four partners into
-          // the slots (the same i, f, w, mask and terms as the gather-form
-          // adjoint); a column's slot is flushed into the data cube (global
-          // atomic adds of its nonzero samples) in the group after its last
-          // step and zeroed there for the column that takes the slot next
-          // (the double ring: that column's first step is a group later).
+          // ---- adjoint, column by column: each wave takes whole columns
+          // (side, o) and adds the terms of every output crossline of the
+          // workgroup that reads the column -- crossline v at offset dk = o -
+          // v on side 0, o - (W - 1 - v) on side 1, the same (i, f, w, mask)
+          // as the gather-form adjoint: d[i] += w (1 - f) m, d[i + 1] += w f m
+          // -- into a window of its own (samples lo .. lo + kWinPairs - 1 of
+          // the column's trace in both ring rows; lo as in the table, hi - lo
+          // <= kWinPairs - 2), then flushes the window into the data cube
+          // (global atomic adds of its nonzero samples) and zeroes it.  No
+          // window is shared between waves, so the sweep needs no barrier.
+          // Within a wave, lanes with the same i collide: each round, the
+          // lanes still pending claim their i in the wave's claim row (one
+          // writer per address wins), the winners add (sample i, then i +
+          // 1), the others retry.
           using V2 = float __attribute__((ext_vector_type(2)));
           const int* const wid = reinterpret_cast<const int*>(smem + PL::kWid);
-          const int jsh = two ? 2 : 1;  // log2 of the jobs per column
-          auto job_at = [&](int oa, int nc, int jj) {
-            return StageJob{oa + (jj >> jsh), (jj >> (jsh - 1)) & 1, jj & (jsh - 1), jj < (nc << jsh)};
-          };
-          auto tab = [&](int side, int o) { return meta[side * PL::kCols + o]; };
-          // flush column jb.o of jb.side, ring row jb.rr: lane l holds the
-          // column's samples lo + 2l, lo + 2l + 1
-          auto flush = [&](const StageJob& jb) {
-            if (!jb.ok) return;
-            const int sl = PL::slot(jb.side, jb.o);
-            char* const cell = smem + sl + jb.rr * PL::kRow + lane * 8;
-            const V2 v = *reinterpret_cast<const V2*>(cell);
-            *reinterpret_cast<V2*>(cell) = V2{0.0f, 0.0f};
-            const int k = jb.side == 0 ? kbase + jb.o : kbase + (W - 1) - jb.o;
-            if (k < xd_lo || k >= xd_hi) return;  // outside the data: nothing scattered there counts
-            const int lo = (sl - tab(jb.side, jb.o)) / kB;
-            const int n = wid[jb.side * PL::kCols + jb.o];
-            float* const dst = a.adj_out +
-                               ((size_t)(jb.rr == 0 ? r0 : rm) * a.n_xd + (k - xd_lo)) * a.n_t + lo +
-                               2 * lane;
-            if (2 * lane < n && v.x != 0.0f) unsafeAtomicAdd(dst, v.x);
-            if (2 * lane + 1 < n && v.y != 0.0f) unsafeAtomicAdd(dst + 1, v.y);
-          };
-          constexpr int kJobs = (4 * G + W - 1) / W;  // flush jobs per wave per group, at most
-          using Tab = StageTab<G, 1>;
-          auto read_tab = [&](int dk0, Tab& t) {
-#pragma unroll
-            for (int u = 0; u < G; ++u) {
-              const int dk = dk0 + u;
-              t.r2[u] = r2tab[dk];
-              t.bR[u] = tab(0, w + dk);
-              t.bL[u] = tab(1, (W - 1 - w) + dk);
-            }
-          };
-          // Plain read-add-write of the two samples i, i + 1 of a window
-          // (LDS FP32 atomics run at about one lane per cycle).  Between
-          // barriers every window belongs to one wave (one step per group:
-          // at a step the waves touch distinct columns), so only lanes of
-          // one wave with the same i collide: each round, the lanes still
-          // pending claim their i in the wave's claim row (one writer per
-          // address wins), the winners add, the others retry.  i mod
-          // kWinPairs identifies i: a wave's i at one step lie in one window.
-          static_assert(G == 1, "one step per group: windows private to one wave between barriers");
           using LdsInt = volatile __attribute__((address_space(3))) int;  // LDS, not flat
           LdsInt* const claim = (LdsInt*)(smem + PL::kClaim) + w * kWinPairs;
-          // adds c at byte offset `off` of the partners' cells (addresses
-          // ar, al in ring row 0, + kRow in ring row 1): all reads, then all
-          // writes (the partners' windows are distinct slots)
-          auto rmw = [&](auto nr_c, bool one, int ar, int al, int off, float c) {
+          const int wslot = w * PL::kSlot;  // this wave's window in ring row 0 (row 1: + kRow)
+          auto run_cols = [&](auto nr_c) {
             constexpr int NR = decltype(nr_c)::value;
-            float* const p0 = reinterpret_cast<float*>(smem + ar + off);
-            float* const p1 = reinterpret_cast<float*>(smem + al + off);
-            float* const p2 = reinterpret_cast<float*>(smem + ar + PL::kRow + off);
-            float* const p3 = reinterpret_cast<float*>(smem + al + PL::kRow + off);
-            const float x0 = *p0;
-            const float x1 = one ? 0.0f : *p1;
-            const float x2 = NR == 2 ? *p2 : 0.0f;
-            const float x3 = (NR == 2 && !one) ? *p3 : 0.0f;
-            *p0 = x0 + c;
-            if (!one) *p1 = x1 + c;
-            if constexpr (NR == 2) {
-              *p2 = x2 + c;
-              if (!one) *p3 = x3 + c;
-            }
-          };
-          // steps dk0 + u, u < n (FULL: all G); FIRST: dk = 0 has one column;
-          // MASK = false: every valid lane is inside its aperture
-          auto scatter = [&](auto nr_c, auto full_c, auto first_c, auto mask_c, int dk0, int n,
-                             const Tab& t) {
-            constexpr bool FULL = decltype(full_c)::value;
-            constexpr bool FIRST = decltype(first_c)::value;
-            constexpr bool MASK = decltype(mask_c)::value;
+            auto rmw_rows = [&](int ar, float cc) {
+              float* const p0 = reinterpret_cast<float*>(smem + ar);
+              float* const p2 = reinterpret_cast<float*>(smem + ar + PL::kRow);
+              const float x0 = *p0;
+              const float x2 = NR == 2 ? *p2 : 0.0f;
+              *p0 = x0 + cc;
+              if constexpr (NR == 2) *p2 = x2 + cc;
+            };
+            for (int job = w; job < 2 * nO; job += W) {
+              const int side = job & 1, o = job >> 1;
+              const int lo = uniform_i((PL::slot(side, o) - meta[side * PL::kCols + o]) / 4);
+              const int bse = wslot - lo * 4;
 #pragma unroll
-            for (int u = 0; u < G; ++u) {
-              if (FULL || u < n) {
-                const bool one = FIRST && u == 0;
-                const float r2 = t.r2[u];
-                const TauW tw = tau_weight(p, r2, AMP);
+              for (int v = 0; v < W; ++v) {
+                const int dk = side == 0 ? o - v : o - (W - 1 - v);
+                if (dk < side || dk > kmax_wg) continue;
+                const float r2 = r2tab[dk];
+                const TauW tw = tau_weight(pw[v], r2, AMP);
                 const float q = tw.tau * a.inv_dt;
                 const int i = (int)q;
-                const bool on = (!MASK || r2 <= p.R2) && i < nt1;
+                const bool on = r2 <= pw[v].R2 && i < nt1;
                 const float f = __builtin_amdgcn_fractf(q);
-                const float wm = tw.w * p.mval;
+                const float wm = tw.w * pw[v].mval;
                 const float c0 = wm - wm * f, c1 = wm * f;
-                int ar, al;
-                asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(ar) : "v"(i), "v"(t.bR[u]));
-                asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(al) : "v"(i), "v"(t.bL[u]));
+                int ar;
+                asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(ar) : "v"(i), "v"(bse));
                 // only lanes with a term take part (i of the others may lie
                 // outside the window)
                 const int ci = i & (kWinPairs - 1);
@@ -1024,48 +995,38 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                   asm volatile("" ::: "memory");  // the previous round's adds come first
                   if (pending) claim[ci] = lane;
                   const bool win = pending && claim[ci] == lane;
-                  // samples i first, then i + 1 (the winner at i + 1 adds there
-                  // too): within each phase the winners' cells are distinct
-                  if (win) rmw(nr_c, one, ar, al, 0, c0);
+                  // samples i first, then i + 1 (the winner at i + 1 adds
+                  // there too): within each phase the winners' cells are
+                  // distinct
+                  if (win) rmw_rows(ar, c0);
                   asm volatile("" ::: "memory");
-                  if (win) rmw(nr_c, one, ar, al, 4, c1);
+                  if (win) rmw_rows(ar + 4, c1);
                   pending = pending && !win;
+                }
+              }
+              asm volatile("" ::: "memory");  // the window is complete
+              // flush: lane l holds the column's samples lo + 2l, lo + 2l + 1
+              const int k = side == 0 ? kbase + o : kbase + (W - 1) - o;
+              const int n = wid[side * PL::kCols + o];
+#pragma unroll
+              for (int rr = 0; rr < NR; ++rr) {
+                char* const cell = smem + wslot + rr * PL::kRow + lane * 8;
+                const V2 val = *reinterpret_cast<const V2*>(cell);
+                *reinterpret_cast<V2*>(cell) = V2{0.0f, 0.0f};
+                if (k >= xd_lo && k < xd_hi) {  // outside the data: nothing scattered there counts
+                  float* const dst = a.adj_out +
+                                     ((size_t)(rr == 0 ? r0 : rm) * a.n_xd + (k - xd_lo)) * a.n_t +
+                                     lo + 2 * lane;
+                  if (2 * lane < n && val.x != 0.0f) unsafeAtomicAdd(dst, val.x);
+                  if (2 * lane + 1 < n && val.y != 0.0f) unsafeAtomicAdd(dst + 1, val.y);
                 }
               }
             }
           };
-          int flushed = 0;  // columns [0, flushed) are flushed
-          auto run_group = [&](auto nr_c, auto full_c, auto first_c, auto mask_c, int dk0, Tab& t) {
-            if (dk0 >= G) {  // the previous group's G columns are complete
-#pragma unroll
-              for (int qq = 0; qq < kJobs; ++qq) flush(job_at(dk0 - G, G, w + qq * W));
-              flushed = dk0;
-            }
-            scatter(nr_c, full_c, first_c, mask_c, dk0, min(G, kmax_wg + 1 - dk0), t);
-            read_tab(dk0 + G, t);
-            __syncthreads();  // the group's sums are in; the zeroed slots are visible
-          };
-          auto run_row = [&](auto nr_c) {
-            using T = std::integral_constant<bool, true>;
-            using F = std::integral_constant<bool, false>;
-            Tab t;
-            read_tab(0, t);
-            if (kmax_wg + 1 >= G)
-              run_group(nr_c, T{}, T{}, T{}, 0, t);
-            else
-              run_group(nr_c, F{}, T{}, T{}, 0, t);
-            int dk0 = G;
-            for (; dk0 + G <= min(kmax_wg, kin) + 1; dk0 += G) run_group(nr_c, T{}, F{}, F{}, dk0, t);
-            for (; dk0 + G <= kmax_wg + 1; dk0 += G) run_group(nr_c, T{}, F{}, T{}, dk0, t);
-            if (dk0 <= kmax_wg) run_group(nr_c, F{}, F{}, T{}, dk0, t);
-            // the remaining columns [flushed, nO)
-            const int nc = nO - flushed;
-            for (int jj = w; jj < (nc << jsh); jj += W) flush(job_at(flushed, nc, jj));
-          };
           if (two)
-            run_row(std::integral_constant<int, 2>{});
+            run_cols(std::integral_constant<int, 2>{});
           else
-            run_row(std::integral_constant<int, 1>{});
+            run_cols(std::integral_constant<int, 1>{});
           continue;
         }
       } else if (!fallback) {
@@ -1697,11 +1658,11 @@ hipError_t launch_lds_wg(const MigArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-// staged adjoint: 4 output crosslines per workgroup, one offset per group (on
-// the 201 x 201 x 751 least-squares cube 117 ms, against 178 ms with 2 and
-// 142 ms with 8 crosslines)
+// staged adjoint: 8 output crosslines per workgroup (each column window
+// collects the terms of all of them before its one flush; on the 201 x 201 x
+// 751 least-squares cube 68.6 ms, against 96.4 ms with 4 and 163 ms with 2)
 hipError_t launch_adj_lds(const MigArgs& a, hipStream_t stream) {
-  constexpr int W = 4, G = 1;
+  constexpr int W = 8, G = 1;
   const int n_tiles = (a.n_t + kWave - 1) / kWave;
   const int n_jgroups = (a.n_out_xl + W - 1) / W;
   const long long per_tile = (long long)a.n_out_il * n_jgroups;
