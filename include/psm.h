@@ -235,9 +235,9 @@ int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
  * 4) = crossline offsets per staged group when on = 1.  The staged form
  * applies to whole-volume calls and per-inline calls with contiguous rows;
  * other calls use the gather form.  The switch also selects the adjoint's
- * form (psm_demigrate_volume): staged (LDS accumulator windows, 4 crosslines
- * x 1 offset per group) unless on = 0.  Results do not depend on the form
- * beyond FP32 summation order. */
+ * form (psm_demigrate_volume): staged (LDS accumulator windows, one per
+ * column and wave, 8 output crosslines per workgroup) unless on = 0.
+ * Results do not depend on the form beyond FP32 summation order. */
 int psm_set_staging(psm_ctx* ctx, int on, int steps);
 int psm_get_staging(psm_ctx* ctx, int* on, int* steps);
 
