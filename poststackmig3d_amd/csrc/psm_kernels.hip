@@ -95,6 +95,15 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
   return v;
 }
 
+// Lane l receives v of lane l - 1 (lane 0 receives 0): DPP wave_shr:1.
+__device__ __forceinline__ int lane_prev(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
+}
+// Lane l receives v of lane l + 1 (lane 63 receives 0): DPP wave_shl:1.
+__device__ __forceinline__ int lane_next(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true);
+}
+
 __device__ __forceinline__ float uniform_f(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
@@ -969,6 +978,24 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
               *p0 = x0 + cc;
               if constexpr (NR == 2) *p2 = x2 + cc;
             };
+            // cells i (w0: add v0) and i + 1 (w1: add v1) of both rows, for
+            // lanes whose cells no other lane writes (any order of the reads
+            // and writes is then safe)
+            auto rmw_own = [&](int ar, bool w0, float v0, bool w1, float v1) {
+              if (!(w0 || w1)) return;
+              float* const p0 = reinterpret_cast<float*>(smem + ar);
+              float* const p2 = reinterpret_cast<float*>(smem + ar + PL::kRow);
+              const float x0 = p0[0], y0 = p0[1];
+              const float x2 = NR == 2 ? p2[0] : 0.0f, y2 = NR == 2 ? p2[1] : 0.0f;
+              if (w0) {
+                p0[0] = x0 + v0;
+                if constexpr (NR == 2) p2[0] = x2 + v0;
+              }
+              if (w1) {
+                p0[1] = y0 + v1;
+                if constexpr (NR == 2) p2[1] = y2 + v1;
+              }
+            };
             for (int job = w; job < 2 * nO; job += W) {
               const int side = job & 1, o = job >> 1;
               const int lo = uniform_i((PL::slot(side, o) - meta[side * PL::kCols + o]) / 4);
@@ -987,21 +1014,53 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                 const float c0 = wm - wm * f, c1 = wm * f;
                 int ar;
                 asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(ar) : "v"(i), "v"(bse));
-                // only lanes with a term take part (i of the others may lie
-                // outside the window)
-                const int ci = i & (kWinPairs - 1);
-                bool pending = on;
-                while (__any(pending)) {
-                  asm volatile("" ::: "memory");  // the previous round's adds come first
-                  if (pending) claim[ci] = lane;
-                  const bool win = pending && claim[ci] == lane;
-                  // samples i first, then i + 1 (the winner at i + 1 adds
-                  // there too): within each phase the winners' cells are
-                  // distinct
-                  if (win) rmw_rows(ar, c0);
-                  asm volatile("" ::: "memory");
-                  if (win) rmw_rows(ar + 4, c1);
-                  pending = pending && !win;
+                // Fast path: i nondecreasing over the lanes with at most two
+                // lanes per value (the usual case: q advances by t0 / tau >=
+                // 1/2 per lane at apertures up to 60 deg).  A run of lanes
+                // with equal i sums its terms in its last lane, which alone
+                // writes cell i (the run's c0 plus the previous run's c1 when
+                // that run's i is i - 1) and cell i + 1 (the run's c1) unless
+                // the next run's i is i + 1 -- that run adds it instead.  A
+                // run writes only when it or the run below it has a term, so
+                // every cell written lies inside the window.
+                const int ip = lane_prev(i), ipp = lane_prev(ip), in_ = lane_next(i);
+                const bool dupp = lane > 0 && i == ip;
+                if (!__any((lane > 0 && i < ip) || (dupp && lane > 1 && ip == ipp))) {
+                  const bool dupn = lane < kWave - 1 && in_ == i;
+                  const float c0v = on ? c0 : 0.0f, c1v = on ? c1 : 0.0f;
+                  const float c0p = __int_as_float(lane_prev(__float_as_int(c0v)));
+                  const float c1p = __int_as_float(lane_prev(__float_as_int(c1v)));
+                  const int onp = lane_prev(on ? 1 : 0);
+                  // run sums and flag, complete in the run's last lane
+                  const float s0 = dupp ? c0v + c0p : c0v;
+                  const float s1 = dupp ? c1v + c1p : c1v;
+                  const int ron = (on || (dupp && onp != 0)) ? 1 : 0;
+                  // the previous run ends at lane l - 1, or l - 2 when lane
+                  // l - 1 belongs to this run
+                  const float s1p = __int_as_float(lane_prev(__float_as_int(s1)));
+                  const float s1pp = __int_as_float(lane_prev(__float_as_int(s1p)));
+                  const int ronp = lane_prev(ron), ronpp = lane_prev(ronp);
+                  const bool take = (dupp ? ronpp : ronp) != 0 && (dupp ? ipp : ip) + 1 == i;
+                  const bool last = !dupn;
+                  rmw_own(ar, last && (ron != 0 || take), take ? s0 + (dupp ? s1pp : s1p) : s0,
+                          last && ron != 0 && in_ != i + 1, s1);
+                } else {
+                  // only lanes with a term take part (i of the others may lie
+                  // outside the window)
+                  const int ci = i & (kWinPairs - 1);
+                  bool pending = on;
+                  while (__any(pending)) {
+                    asm volatile("" ::: "memory");  // the previous round's adds come first
+                    if (pending) claim[ci] = lane;
+                    const bool win = pending && claim[ci] == lane;
+                    // samples i first, then i + 1 (the winner at i + 1 adds
+                    // there too): within each phase the winners' cells are
+                    // distinct
+                    if (win) rmw_rows(ar, c0);
+                    asm volatile("" ::: "memory");
+                    if (win) rmw_rows(ar + 4, c1);
+                    pending = pending && !win;
+                  }
                 }
               }
               asm volatile("" ::: "memory");  // the window is complete
