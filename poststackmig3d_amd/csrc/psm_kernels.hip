@@ -1427,9 +1427,12 @@ void migrate_lds_kernel(MigArgs a, int n_tiles, int n_jgroups, int per_tile_pad)
 }
 
 // The staged adjoint over the same grid as migrate_lds_kernel (FP32 data).
+// (8-wave workgroups: registers capped at 80 so that three of them share a
+// CU, as the LDS allows; 89 registers left room for two, 3 % slower)
 template <int W, int G, bool AMP>
-__global__ __launch_bounds__(kWave * W) void demigrate_lds_kernel(MigArgs a, int n_tiles,
-                                                                  int n_jgroups, int per_tile_pad) {
+__global__ __launch_bounds__(kWave * W)
+__attribute__((amdgpu_waves_per_eu(W == 8 ? 6 : 1)))
+void demigrate_lds_kernel(MigArgs a, int n_tiles, int n_jgroups, int per_tile_pad) {
   __shared__ __attribute__((aligned(16))) char smem[LdsPlan<PSM_STORE_F32, W, G, kAdjoint>::kBytes];
   const int g = blockIdx.x;
   const int tile = n_tiles - 1 - g / per_tile_pad;
