@@ -142,6 +142,26 @@ int psm_migrate_volume(psm_ctx* ctx, const psm_survey* survey,
                        int where);
 
 /*
+ * psm_migrate_volume without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  data, vel and out are device memory; the call enqueues on
+ * the context's stream and returns once the work is queued, and its result
+ * equals psm_migrate_volume's with PSM_DEVICE.  The pair-layout cube, the
+ * per-trace scales, the output-inline tasks and the t0 table live in a
+ * static 64 MiB workspace that the library holds on every device it runs on
+ * until the process ends: a survey that needs more than that (about
+ * 8 B x (n_il + 1) x (n_xl + 1) x n_t with FP32 storage, half that with
+ * FP16/BF16 storage, e.g. 101 x 101 x 501 but not 201 x 201 x 751 in FP32)
+ * is rejected with PSM_ERR_ARG.  Every call on one device shares that
+ * workspace: calls that overlap on one GPU (from any context or stream) race
+ * and must be serialised by the caller.  Kernel timing
+ * (psm_enable_timing) is not recorded by this call.
+ */
+int psm_migrate_volume_noalloc(psm_ctx* ctx, const psm_survey* survey,
+                               const float* data, const float* vel, float* out);
+
+/*
  * Adjoint of psm_migrate_volume (Kirchhoff demigration / modelling; no
  * reference counterpart): image [n_out_il][n_out_xl][n_t] -> data
  * [n_il][n_xl][n_t].  Walks the same pairs with the same sample index,

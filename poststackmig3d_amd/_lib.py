@@ -84,6 +84,8 @@ SIGNATURES = [
     ("psm_plan_free", None, [C.POINTER(Plan)]),
     ("psm_migrate_volume", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_int]),
+    ("psm_migrate_volume_noalloc", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p,
+                                             C.c_void_p, C.c_void_p]),
     ("psm_count_pairs", C.c_int, [C.c_void_p, C.POINTER(Survey), C.c_void_p, C.c_int,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p,
                                   C.c_void_p]),

@@ -20,7 +20,38 @@
 #include "psm.h"
 #include "psm_internal.h"
 
+// Workspace of psm_migrate_volume_noalloc: the pair-layout cube, the per-trace
+// scales, the output-inline tasks and the t0 table of one call, carved out of
+// this array instead of allocated.  Each device holds its own copy for the
+// life of the process.  Host code never uses the array's name as an address:
+// its device address on each device comes from hipGetSymbolAddress
+// (resolve_scratch) and is kept in g_scratch.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices];  // device address of `scratch` per device; nullptr: unresolved
+
+// Looks up the device address of `scratch` on `device` (which must be the
+// current device) the first time it is asked for that device.
+hipError_t resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch[device] = p;
+  return e;
+}
+
+void* scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[device];
+}
 
 // Grow-only device allocation owned by a context.
 struct DevBuf {
@@ -254,6 +285,9 @@ int psm_create(psm_ctx** out, int device) {
     return PSM_ERR_HIP;
   }
   c->stream = c->own_stream;
+  // once per device: the device address of the no-allocation workspace (a
+  // failure only disables psm_migrate_volume_noalloc, which reports it)
+  (void)resolve_scratch(device);
   *out = c;
   return PSM_OK;
 }
@@ -406,7 +440,7 @@ namespace {
 //   kCount    in unused,       out unused; counts (and count images)
 int volume_impl(psm_ctx* c, const psm_survey* s, const float* in, const float* vel, float* out,
                 int where, int mode, double* p_ap = nullptr, double* p_g = nullptr,
-                float* ap_image = nullptr, float* g_image = nullptr) {
+                float* ap_image = nullptr, float* g_image = nullptr, bool noalloc = false) {
   const bool count = mode == psm::kCount;
   const bool adjoint = mode == psm::kAdjoint;
   const bool gathers = mode == psm::kForward || mode == psm::kTangent;
@@ -435,8 +469,43 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* in, const float* v
                           plan.out_to_in_il[io]};
     extent = std::max(extent, std::max(tasks[io].b1 - 1 - tasks[io].c, tasks[io].c - tasks[io].b0));
   }
-  PSM_HIP(c, upload(c->tasks, tasks.data(), tasks.size() * sizeof(psm::Task), st));
-  PSM_HIP(c, upload(c->t0, plan.t0_times, nt * sizeof(float), st));
+  if (((size_t)s->n_xl + 1) * nt * sizeof(float2) >= (size_t)INT32_MAX)
+    return fail(c, PSM_ERR_ARG, "one inline of the survey exceeds 2 GiB");
+  const size_t pairs_bytes = psm::pairs_bytes(storage, (int)s->n_il, (int)s->n_xl, (int)nt);
+  const size_t scale_bytes = (s->n_il + 1) * (s->n_xl + 1) * sizeof(float);
+  const size_t tasks_bytes = tasks.size() * sizeof(psm::Task);
+
+  // Workspace: the context's grow-only buffers, or (noalloc) consecutive
+  // 256-byte aligned pieces of the static `scratch` array at this device's
+  // address: tasks, t0 table, scales, pairs.
+  psm::Task* w_tasks;
+  float* w_t0;
+  float* w_scale = nullptr;
+  void* w_pairs = nullptr;
+  if (noalloc) {
+    char* const base = static_cast<char*>(scratch_address(c->device));
+    if (!base)
+      return fail(c, PSM_ERR_HIP, "the no-allocation workspace has no device address on device %d",
+                  c->device);
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t need = up(tasks_bytes) + up(nt * sizeof(float)) + up(scale_bytes) + up(pairs_bytes);
+    if (need > kScratchBytes)
+      return fail(c, PSM_ERR_ARG,
+                  "survey %zu x %zu x %zu needs %zu bytes of workspace; "
+                  "psm_migrate_volume_noalloc holds at most %zu",
+                  s->n_il, s->n_xl, nt, need, kScratchBytes);
+    w_tasks = reinterpret_cast<psm::Task*>(base);
+    w_t0 = reinterpret_cast<float*>(base + up(tasks_bytes));
+    w_scale = reinterpret_cast<float*>(base + up(tasks_bytes) + up(nt * sizeof(float)));
+    w_pairs = base + up(tasks_bytes) + up(nt * sizeof(float)) + up(scale_bytes);
+    PSM_HIP(c, hipMemcpyAsync(w_tasks, tasks.data(), tasks_bytes, hipMemcpyHostToDevice, st));
+    PSM_HIP(c, hipMemcpyAsync(w_t0, plan.t0_times, nt * sizeof(float), hipMemcpyHostToDevice, st));
+  } else {
+    PSM_HIP(c, upload(c->tasks, tasks.data(), tasks_bytes, st));
+    PSM_HIP(c, upload(c->t0, plan.t0_times, nt * sizeof(float), st));
+    w_tasks = c->tasks.as<psm::Task>();
+    w_t0 = c->t0.as<float>();
+  }
 
   // device views of the inputs and outputs
   const float* d_vel = vel;
@@ -454,13 +523,13 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* in, const float* v
       d_out = c->out.as<float>();
     }
   }
-  if (((size_t)s->n_xl + 1) * nt * sizeof(float2) >= (size_t)INT32_MAX)
-    return fail(c, PSM_ERR_ARG, "one inline of the survey exceeds 2 GiB");
-  if (gathers) {
-    PSM_ALLOC(c, c->pairs, psm::pairs_bytes(storage, (int)s->n_il, (int)s->n_xl, (int)nt));
-    PSM_ALLOC(c, c->scale, (s->n_il + 1) * (s->n_xl + 1) * sizeof(float));
+  if (gathers && !noalloc) {
+    PSM_ALLOC(c, c->pairs, pairs_bytes);
+    PSM_ALLOC(c, c->scale, scale_bytes);
+    w_pairs = c->pairs.p;
+    w_scale = c->scale.as<float>();
   }
-  const bool timed = c->timing && mode == psm::kForward;
+  const bool timed = c->timing && mode == psm::kForward && !noalloc;
   c->timed = false;
   if (timed) {
     for (hipEvent_t& e : c->ev)
@@ -468,19 +537,19 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* in, const float* v
     PSM_HIP(c, hipEventRecord(c->ev[0], st));
   }
   if (gathers)
-    PSM_HIP(c, psm::launch_pack(storage, d_in, c->pairs.p, c->scale.as<float>(), (int)s->n_il,
-                                (int)s->n_xl, (int)nt, st));
+    PSM_HIP(c, psm::launch_pack(storage, d_in, w_pairs, w_scale, (int)s->n_il, (int)s->n_xl,
+                                (int)nt, st));
   if (timed) PSM_HIP(c, hipEventRecord(c->ev[1], st));
 
   psm::MigArgs a{};
-  a.xp = c->pairs.p;
-  a.scale = c->scale.as<float>();
+  a.xp = w_pairs;
+  a.scale = w_scale;
   a.n_rows = (int)s->n_il;
   a.storage = storage;
   a.vel = d_vel;
   a.out = gathers ? d_out : nullptr;
-  a.t0 = c->t0.as<float>();
-  a.tasks = c->tasks.as<psm::Task>();
+  a.t0 = w_t0;
+  a.tasks = w_tasks;
   a.row_pos = nullptr;
   a.n_t = (int)nt;
   a.n_xd = (int)s->n_xl;
@@ -557,6 +626,14 @@ int psm_migrate_volume(psm_ctx* c, const psm_survey* s, const float* data, const
   if (!c) return PSM_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
   return volume_impl(c, s, data, vel, out, where, psm::kForward);
+}
+
+int psm_migrate_volume_noalloc(psm_ctx* c, const psm_survey* s, const float* data,
+                               const float* vel, float* out) {
+  if (!c) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  return volume_impl(c, s, data, vel, out, PSM_DEVICE, psm::kForward, nullptr, nullptr, nullptr,
+                     nullptr, true);
 }
 
 int psm_demigrate_volume(psm_ctx* c, const psm_survey* s, const float* image, const float* vel,
