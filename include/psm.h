@@ -60,6 +60,12 @@ const char* psm_last_error(const psm_ctx* ctx);
  * (default) stream) for every later call on this context.  A new context
  * uses a stream of its own. */
 int psm_set_stream(psm_ctx* ctx, void* stream);
+/* The HIP stream later calls on this context enqueue on (hipStream_t as
+ * void*; NULL = the null stream).  The library orders nothing against other
+ * streams: a caller that produces inputs on another stream makes this one
+ * wait for that work first (an event, or psm_set_stream to that stream), and
+ * waits on this one before reading the outputs of a PSM_DEVICE call. */
+int psm_get_stream(psm_ctx* ctx, void** stream);
 int psm_synchronize(psm_ctx* ctx);
 
 /*
@@ -121,7 +127,10 @@ typedef struct psm_plan {
 } psm_plan;
 
 /* vel_probe: the velocity cube (host or device per `where`) used when
- * survey->max_velocity <= 0; may be NULL otherwise. */
+ * survey->max_velocity <= 0; may be NULL otherwise.  A device cube is read
+ * by a synchronous copy on the null stream: work on other streams that
+ * writes it must be complete.  (The volume calls probe on their context's
+ * stream instead.) */
 int psm_survey_plan(const psm_survey* survey, const float* vel_probe, int where,
                     psm_plan* plan);
 void psm_plan_free(psm_plan* plan);

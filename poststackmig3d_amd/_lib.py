@@ -75,6 +75,7 @@ SIGNATURES = [
     ("psm_destroy", None, [C.c_void_p]),
     ("psm_last_error", C.c_char_p, [C.c_void_p]),
     ("psm_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("psm_get_stream", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("psm_synchronize", C.c_int, [C.c_void_p]),
     ("psm_kirchhoff_inline", C.c_int, [
         C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p,
@@ -170,6 +171,13 @@ class Context:
     def set_stream(self, stream_ptr: int | None):
         """Runs later calls on this HIP stream (0/None = the null stream)."""
         self.check(self.lib.psm_set_stream(self.h, C.c_void_p(stream_ptr or 0)), "psm_set_stream")
+
+    def stream_ptr(self) -> int:
+        """The HIP stream (as an integer handle, 0 = null stream) this
+        context's calls enqueue on."""
+        s = C.c_void_p()
+        self.check(self.lib.psm_get_stream(self.h, C.byref(s)), "psm_get_stream")
+        return s.value or 0
 
     def synchronize(self):
         self.check(self.lib.psm_synchronize(self.h), "psm_synchronize")

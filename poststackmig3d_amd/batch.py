@@ -15,7 +15,7 @@ import ctypes as C
 
 from . import _lib
 from ._lib import STORAGE, BatchItem, PsmError
-from .kernel import SurveyParams, _ptr_where, PSM_DEVICE
+from .kernel import SurveyParams, _ptr_where, _TorchOrder, PSM_DEVICE
 
 
 class Batch:
@@ -38,6 +38,7 @@ class Batch:
             arr[k].data, arr[k].vel, arr[k].out = ptrs
             arr[k].alpha = float(alpha)
         self._items = arr
+        self._tensors = [t for it in items for t in it[1:4]]
         h = C.c_void_p()
         self.ctx.check(self.ctx.lib.psm_batch_create(self.ctx.h, arr, len(items), STORAGE[storage],
                                                      C.byref(h)), "psm_batch_create")
@@ -54,9 +55,12 @@ class Batch:
         return w.value, b.value, ms.value
 
     def run(self, timed: bool = False):
+        """Replays the batch graph on the context's stream, ordered after the
+        work torch has queued and before torch's later work."""
         ms = C.c_float()
-        self.ctx.check(self.ctx.lib.psm_batch_run(self.h, C.byref(ms) if timed else None),
-                       "psm_batch_run")
+        with _TorchOrder(self.ctx, *self._tensors[:1]):
+            self.ctx.check(self.ctx.lib.psm_batch_run(self.h, C.byref(ms) if timed else None),
+                           "psm_batch_run")
         return ms.value if timed else None
 
     def close(self):
@@ -69,3 +73,42 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+SCAN_ALPHAS = 16
+SCAN_SUBVOLS = 8
+SCAN_GEOMETRY = dict(dt=0.002, inline_step=25.0, crossline_step=25.0, inline_padding=0,
+                     crossline_padding=0, angle_aperture=30.0, amp_correction=True)
+
+
+def velocity_scan(device, n=(301, 301, 1001)):
+    """The velocity-scan workload: the point-diffractor cube (V = 2000 m/s,
+    25 Hz Ricker apex at the centre, 10 % noise; 30 deg, amplitude correction
+    on, no padding) at n = 301 x 301 x 1001, migrated under 16 scales
+    alpha = 0.85 .. 1.15 of its Vrms, plus 8 centred sub-volumes from
+    64 x 64 x 501 to the whole cube at alpha = 1.  Returns (items, base,
+    vbase): items = [(SurveyParams, data, vel, out, alpha)] with device
+    tensors and empty outputs (the form Batch takes), base / vbase the host
+    cube and velocity the items are cut from."""
+    import numpy as np
+    import torch
+
+    from . import synth
+    base_np, vbase_np = synth.point_diffractor(*n)
+    base = torch.from_numpy(base_np).to(device)
+    vbase = torch.from_numpy(vbase_np).to(device)
+
+    def item(d, v, alpha):
+        prm = SurveyParams.for_cube(tuple(d.shape), **SCAN_GEOMETRY)
+        out = torch.empty(tuple(d.shape), dtype=torch.float32, device=device)  # no padding
+        return (prm, d, v, out, float(alpha))
+
+    items = [item(base, vbase, a) for a in np.linspace(0.85, 1.15, SCAN_ALPHAS)]
+    sizes = np.linspace(64, n[0], SCAN_SUBVOLS).round().astype(int)
+    lengths = np.linspace(501, n[2], SCAN_SUBVOLS).round().astype(int)
+    for a, t in zip(sizes, lengths):
+        o = (n[0] - a) // 2
+        items.append(item(base[o:o + a, o:o + a, :t].contiguous(),
+                          vbase[o:o + a, o:o + a, :t].contiguous(), 1.0))
+    return items, base_np, vbase_np
+

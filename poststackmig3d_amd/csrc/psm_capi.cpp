@@ -207,7 +207,12 @@ int plan_impl(psm_ctx* c, const psm_survey* s, const float* vel_probe, int where
         const size_t off = (a * s->n_xl + b) * s->n_t + (s->n_t - 1);
         float v = 0.0f;
         if (where == PSM_DEVICE) {
-          hipError_t e = hipMemcpy(&v, vel_probe + off, sizeof(float), hipMemcpyDeviceToHost);
+          // on the context's stream, so the probe sees what earlier work on
+          // that stream (or work it was made to wait for) wrote into vel
+          hipError_t e = c ? hipMemcpyAsync(&v, vel_probe + off, sizeof(float),
+                                            hipMemcpyDeviceToHost, c->stream)
+                           : hipMemcpy(&v, vel_probe + off, sizeof(float), hipMemcpyDeviceToHost);
+          if (e == hipSuccess && c) e = hipStreamSynchronize(c->stream);
           if (e != hipSuccess)
             return fail(c, PSM_ERR_HIP, "velocity probe copy failed: %s", hipGetErrorString(e));
         } else {
@@ -311,6 +316,13 @@ int psm_set_stream(psm_ctx* c, void* stream) {
   if (!c) return PSM_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
   c->stream = static_cast<hipStream_t>(stream);
+  return PSM_OK;
+}
+
+int psm_get_stream(psm_ctx* c, void** stream) {
+  if (!c || !stream) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  *stream = static_cast<void*>(c->stream);
   return PSM_OK;
 }
 
@@ -889,6 +901,10 @@ int batch_set_shape(psm_batch* b, int wpb, int blocks_per_cu) {
     return fail(c, PSM_ERR_ARG, "batch waves_per_block must be 1, 2, 4 or 8");
   if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(c, PSM_ERR_ARG, "bad blocks_per_cu");
   b->staged = c->tune.staging > 0 || (c->tune.staging < 0 && b->storage == PSM_STORE_F32);
+  // the staged form needs every item's pair-layout rows within kLdsMaxRowBytes
+  for (const psm::MigArgs& a : b->args)
+    if ((size_t)(a.n_xd + 1) * a.n_t * (b->storage == PSM_STORE_F32 ? 8 : 4) > psm::kLdsMaxRowBytes)
+      b->staged = false;
   if (b->staged) wpb = psm::kAutoStageWaves;
   b->wpb = wpb;
   b->blocks_per_cu = blocks_per_cu;
@@ -1052,7 +1068,7 @@ int psm_batch_autotune(psm_batch* b, int* waves_per_block, int* blocks_per_cu, f
   const int cand_gather[][2] = {{1, 32}, {2, 16}, {4, 8}, {8, 4}, {2, 8}, {4, 4}};
   const int cand_staged[][2] = {{psm::kAutoStageWaves, kBatchStagedPerCu},
                                 {psm::kAutoStageWaves, kBatchStagedPerCu - 1}};
-  const bool staged = c->tune.staging > 0 || (c->tune.staging < 0 && b->storage == PSM_STORE_F32);
+  const bool staged = b->staged;  // the form batch_set_shape chose (it keeps it per shape)
   const int n_cand = staged ? 2 : 6;
   float best = -1.0f;
   int bw = 4, bb = 0;

@@ -88,6 +88,12 @@ struct Tuning {
 constexpr int kAutoStageWaves = 8;
 constexpr int kAutoStageSteps = 2;
 
+// Bytes a row of the pair layout may span for the staged forms (the row is
+// addressed through a buffer resource with a 32-bit size, and the staging
+// loads mark lanes out of range with offsets just below 2^31); longer rows
+// run the gather form.
+constexpr size_t kLdsMaxRowBytes = (size_t)1 << 30;
+
 // Launchers (psm_kernels.hip).  All enqueue on `stream` and return the HIP
 // status of the launch.
 // Bytes of the pair layout of n_rows x n_xd traces of n_t samples.

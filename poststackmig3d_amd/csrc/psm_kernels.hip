@@ -730,9 +730,6 @@ struct LdsPlan {
   __device__ static int slot(int side, int o) { return side * kSide + (o % kRing) * kSlot; }
 };
 
-// Bytes a row of the pair layout may span for the staged path (the row is
-// addressed through a buffer resource with a 32-bit size).
-constexpr size_t kLdsMaxRowBytes = (size_t)1 << 30;
 
 // One workgroup of the staged form: output crosslines j0 .. j0 + W - 1 of
 // output inline io over t0 tile `tile`; smem holds LdsPlan<ST, W, G,

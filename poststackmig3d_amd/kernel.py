@@ -120,6 +120,36 @@ def _is_torch(x):
     return type(x).__module__.startswith("torch")
 
 
+class _TorchOrder:
+    """Orders a context's device work against torch's current stream around
+    one call on CUDA tensors: the context's stream first waits for what torch
+    has queued (the inputs may still be in flight there), and torch's stream
+    then waits for the call's work (its outputs), so neither side reads
+    unfinished memory whatever streams the two use.  Nothing is done for host
+    arrays (those calls are synchronous) or when both use the same stream."""
+
+    def __init__(self, ctx, *arrays):
+        self.ctx = ctx
+        self.dev = next((a.device for a in arrays if _is_torch(a) and a.is_cuda), None)
+        self.ext = None
+
+    def __enter__(self):
+        if self.dev is None:
+            return self
+        import torch
+        self.cur = torch.cuda.current_stream(self.dev)
+        sp = self.ctx.stream_ptr()
+        if sp != self.cur.cuda_stream:
+            self.ext = torch.cuda.ExternalStream(sp, device=self.dev)
+            self.ext.wait_stream(self.cur)
+        return self
+
+    def __exit__(self, *exc):
+        if self.ext is not None:
+            self.cur.wait_stream(self.ext)
+        return False
+
+
 def _ptr_where(x, name):
     """(pointer, where) for a numpy array or torch tensor."""
     if _is_torch(x):
@@ -155,6 +185,11 @@ def survey_plan(params: SurveyParams, vel=None) -> PlanInfo:
     s = params.to_c()
     p = Plan()
     ptr, where = (None, PSM_HOST) if vel is None else _ptr_where(vel, "vel")
+    if where == PSM_DEVICE and not params.max_velocity > 0:
+        # the probe reads vel through a synchronous copy that no stream
+        # orders: let torch's queued work (which may still write vel) finish
+        import torch
+        torch.cuda.current_stream(vel.device).synchronize()
     rc = lib.psm_survey_plan(C.byref(s), ptr, where, C.byref(p))
     if rc != _lib.PSM_OK:
         raise PsmError(f"psm_survey_plan failed ({rc})")
@@ -196,9 +231,11 @@ def migrate_volume(data, vel, params: SurveyParams, out=None, ctx: _lib.Context 
     if ow != dw:
         raise PsmError("out must live where data lives")
     s = params.to_c()
-    ctx.check(ctx.lib.psm_migrate_volume(ctx.h, C.byref(s), dp, vp, op, dw), "psm_migrate_volume")
-    if sync and dw == PSM_DEVICE:
-        ctx.synchronize()
+    with _TorchOrder(ctx, data, vel, out):
+        ctx.check(ctx.lib.psm_migrate_volume(ctx.h, C.byref(s), dp, vp, op, dw),
+                  "psm_migrate_volume")
+        if sync and dw == PSM_DEVICE:
+            ctx.synchronize()
     return out
 
 
@@ -220,9 +257,10 @@ def _volume_call(fn_name, inp, vel, params, out, out_shape, ctx, sync):
     if ow != iw:
         raise PsmError("out must live where the input lives")
     s = params.to_c()
-    ctx.check(getattr(ctx.lib, fn_name)(ctx.h, C.byref(s), ip, vp, op, iw), fn_name)
-    if sync and iw == PSM_DEVICE:
-        ctx.synchronize()
+    with _TorchOrder(ctx, inp, vel, out):
+        ctx.check(getattr(ctx.lib, fn_name)(ctx.h, C.byref(s), ip, vp, op, iw), fn_name)
+        if sync and iw == PSM_DEVICE:
+            ctx.synchronize()
     return out
 
 
@@ -257,7 +295,8 @@ def dot_f64(x, y, ctx: _lib.Context | None = None) -> float:
     if xw != yw or n != int(np.prod(y.shape)):
         raise PsmError("x and y must have the same size and live in the same memory")
     r = C.c_double()
-    ctx.check(ctx.lib.psm_dot_f64(ctx.h, xp, yp, n, xw, C.byref(r)), "psm_dot_f64")
+    with _TorchOrder(ctx, x, y):
+        ctx.check(ctx.lib.psm_dot_f64(ctx.h, xp, yp, n, xw, C.byref(r)), "psm_dot_f64")
     return r.value
 
 
@@ -280,10 +319,11 @@ def count_pairs(vel, params: SurveyParams, ctx: _lib.Context | None = None, imag
         else:
             ap_img = np.empty(shape, np.float32)
             g_img = np.empty(shape, np.float32)
-    ctx.check(ctx.lib.psm_count_pairs(ctx.h, C.byref(s), vp, vw, C.byref(pap), C.byref(pg),
-                                      None if ap_img is None else _ptr_where(ap_img, "ap")[0],
-                                      None if g_img is None else _ptr_where(g_img, "g")[0]),
-              "psm_count_pairs")
+    with _TorchOrder(ctx, vel, ap_img, g_img):
+        ctx.check(ctx.lib.psm_count_pairs(ctx.h, C.byref(s), vp, vw, C.byref(pap), C.byref(pg),
+                                          None if ap_img is None else _ptr_where(ap_img, "ap")[0],
+                                          None if g_img is None else _ptr_where(g_img, "g")[0]),
+                  "psm_count_pairs")
     if images:
         return pap.value, pg.value, ap_img, g_img
     return pap.value, pg.value

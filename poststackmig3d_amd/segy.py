@@ -65,6 +65,19 @@ def write(path, traces, inlines, crosslines, dt_us=2000, fmt=5, text=None, order
     bh[20:22] = int(n_t).to_bytes(2, "big")
     bh[24:26] = int(fmt).to_bytes(2, "big")
     idx = np.arange(n) if order is None else np.asarray(order)
+    if fmt == 5 and extra_headers is None:
+        # IEEE samples and plain headers: the whole trace block at once
+        body = np.zeros((n, TRACE_HEADER + 4 * n_t), np.uint8)
+        body[:, 114:116] = np.frombuffer(int(n_t).to_bytes(2, "big"), np.uint8)
+        body[:, 116:118] = np.frombuffer(int(dt_us).to_bytes(2, "big"), np.uint8)
+        body[:, 188:192] = np.asarray(inlines, ">i4")[idx].view(np.uint8).reshape(n, 4)
+        body[:, 192:196] = np.asarray(crosslines, ">i4")[idx].view(np.uint8).reshape(n, 4)
+        body[:, TRACE_HEADER:] = traces[idx].astype(">f4").view(np.uint8).reshape(n, 4 * n_t)
+        with open(path, "wb") as f:
+            f.write(bytes(th))
+            f.write(bytes(bh))
+            body.tofile(f)
+        return
     with open(path, "wb") as f:
         f.write(bytes(th))
         f.write(bytes(bh))

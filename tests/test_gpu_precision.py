@@ -2,13 +2,15 @@
 scaling; travel time, weights and accumulation in FP32) against the FP64
 oracle, and the FP32 form for scale.
 
-Tolerances (relative L2 over the whole output):
+Tolerances (relative L2 over the whole output) on a padded, laterally varying
+61 x 57 x 301 cube:
   FP32 storage  <= 1e-5 against the reference FP32 kernel (its own distance
                 to the FP64 oracle, ~5e-5 on this noisy cube, is the float32
                 rounding of the sample index and is printed alongside)
-  FP16 storage  <= 1e-3 against the FP64 oracle
-  BF16 storage  <= 1e-2 against the FP64 oracle (8-bit significand: ~2e-3
-                relative rounding per stored sample)
+  FP16 storage  <= 4e-4 against the FP64 oracle (about twice the 1.7e-4
+                MI355X gives)
+  BF16 storage  <= 3e-3 against the FP64 oracle (8-bit significand: ~2e-3
+                relative rounding per stored sample; MI355X gives 1.3e-3)
 """
 import numpy as np
 import pytest
@@ -22,7 +24,7 @@ from poststackmig3d_amd._lib import Context
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"f32": 1e-5, "f16": 1e-3, "bf16": 1e-2}
+TOL = {"f32": 1e-5, "f16": 4e-4, "bf16": 3e-3}   # MI355X: f16 1.7e-4, bf16 1.3e-3
 
 
 @pytest.fixture(scope="module")
@@ -55,6 +57,50 @@ def test_storage_against_fp64_oracle(case, storage):
         assert rel_l2(got, f32) <= TOL["f32"]
     else:
         assert err <= TOL[storage], err
+
+
+# The precision form of the plain-method case: the 101 x 101 x 501 point
+# diffractor (V = 2000 m/s, 25 Hz Ricker apex at the centre, 10 % noise, 30 deg,
+# amplitude correction on, no padding), the whole image against the FP64
+# oracle.  Bars: relative L2 and the largest per-output error relative to the
+# image's peak, each about twice what MI355X gives (FP16: 2.7e-4 and 7.8e-5,
+# BF16: 2.2e-3 and 7.0e-4; the reference's own FP32 kernel: 2.4e-5 and 7.9e-6).
+PLAIN_TOL = {"f32": (1e-5, 1e-5), "f16": (6e-4, 2e-4), "bf16": (5e-3, 1.5e-3)}
+
+
+@pytest.fixture(scope="module")
+def plain():
+    data, vel = synth.point_diffractor(101, 101, 501, v=2000.0, f=25.0, noise=0.1, seed=1)
+    f64 = O.migrate_volume(data, vel, 0.002, 25.0, 25.0, 0, 0, 30.0, True, precision="f64")
+    f32 = reference_volume(data, vel, 0.002, 25.0, 25.0, 0, 0, 30.0, True)
+    prm = P.SurveyParams.for_cube(data.shape, angle_aperture=30.0)
+    return data, vel, prm, f64, f32
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_plain_method_storage_against_fp64_oracle(plain, storage):
+    data, vel, prm, f64, f32 = plain
+    ctx = Context(0)
+    try:
+        ctx.set_storage(storage)
+        got = P.migrate_volume(data, vel, prm, ctx=ctx)
+    finally:
+        ctx.close()
+    err = rel_l2(got, f64)
+    peak = float(np.abs(f64).max())
+    maxerr = float(np.abs(got.astype(np.float64) - f64).max() / peak)
+    print(f"plain 101x101x501 storage {storage}: rel-L2 vs FP64 {err:.3e}, max |err| / peak "
+          f"{maxerr:.3e}; reference FP32 vs FP64: rel-L2 {rel_l2(f32, f64):.3e}, max "
+          f"{float(np.abs(f32.astype(np.float64) - f64).max() / peak):.3e}")
+    assert np.isfinite(got).all()
+    # the diffraction focuses at the apex (il, xl) = (50, 50), t = 0.5 s
+    assert np.unravel_index(np.argmax(np.abs(got)), got.shape) [:2] == (50, 50)
+    bar_l2, bar_max = PLAIN_TOL[storage]
+    if storage == "f32":
+        assert rel_l2(got, f32) <= bar_l2
+    else:
+        assert err <= bar_l2, err
+        assert maxerr <= bar_max, maxerr
 
 
 def test_f16_per_inline_dropin(case):
