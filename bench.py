@@ -103,8 +103,10 @@ def parse():
                     help="trace storage of the main line (compute stays FP32)")
     ap.add_argument("--kernel", choices=["auto", "staged", "gather"], default="auto",
                     help="forward kernel form: chosen by trace storage (auto), aperture "
-                         "windows staged in LDS (8 crosslines per workgroup, 2 offsets per "
+                         "windows staged in LDS (8 crosslines per workgroup, --stage-steps offsets per "
                          "group, one launch), or gathered through the vector memory path")
+    ap.add_argument("--stage-steps", type=int, choices=[2, 4], default=4,
+                    help="crossline offsets per staged group with --kernel staged")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP16/BF16 storage variants of the production line")
     ap.add_argument("--no-cpu", action="store_true",
@@ -192,7 +194,8 @@ def run_counter_children(args, storage, kernel_form):
         d = tempfile.mkdtemp(prefix="psm_pmc_")
         cmd = [prof, "--pmc", *group, "-d", d, "-o", "run", "--output-format", "csv", "--",
                sys.executable, "-u", os.path.abspath(__file__), "--counter-child",
-               "--workload", args.workload, "--storage", storage, "--kernel", kernel_form]
+               "--workload", args.workload, "--storage", storage, "--kernel", kernel_form,
+               "--stage-steps", str(args.stage_steps)]
         log("counters: " + " ".join(group))
         try:
             r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
@@ -644,7 +647,7 @@ def setup_main(args, torch, P, dev):
     ctx.set_stream(stream.cuda_stream)
     if args.kernel == "staged":
         ctx.set_tuning(8, 0)
-        ctx.set_staging(True, 2)
+        ctx.set_staging(True, args.stage_steps)
     elif args.kernel == "gather":
         ctx.set_staging(False)
     with torch.cuda.stream(stream):

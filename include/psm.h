@@ -259,9 +259,11 @@ int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
  * each aperture trace's time window in LDS once per workgroup of output
  * crosslines (waves_per_block 4 or 8) and gathers from there; on = 0 gathers
  * every sample through the vector memory path; on = -1 (the default) stages
- * FP32 traces with 8 output crosslines per workgroup and 2 crossline offsets
+ * FP32 traces with 8 output crosslines per workgroup and 4 crossline offsets
  * per staged group in one launch, and gathers FP16/BF16 traces.  steps (2 or
- * 4) = crossline offsets per staged group when on = 1.  The staged form
+ * 4) = crossline offsets per staged group when on = 1.  With FP32 traces the
+ * two rows +db, -db of each row pair are staged as one window of summed
+ * pairs (they share every step's travel time and weight).  The staged form
  * applies to whole-volume calls and per-inline calls with contiguous rows;
  * other calls use the gather form.  The switch also selects the adjoint's
  * form (psm_demigrate_volume): staged (LDS accumulator windows, one per

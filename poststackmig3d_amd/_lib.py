@@ -205,7 +205,7 @@ class Context:
         return w.value, r.value
 
 
-    def set_staging(self, on=None, steps: int = 2):
+    def set_staging(self, on=None, steps: int = 4):
         """Forward kernel form: aperture windows staged in LDS (True), gathered
         through the vector memory path (False), or chosen by trace storage
         (None, the default); steps = crossline offsets per staged group (2 or 4)."""

@@ -77,16 +77,18 @@ struct Tuning {
   int staging = -1;          // forward migration: 1 = aperture windows staged in LDS
                              // (workgroups of 4 or 8 waves, contiguous rows), 0 = gathers,
                              // -1 = automatic (kAutoStaging for FP32 traces, else gathers)
-  int stage_steps = 2;       // crossline offsets per staged group (2 or 4)
+  int stage_steps = 4;       // crossline offsets per staged group (2 or 4)
 };
 
 // Automatic forward-kernel choice for FP32 traces: windows staged in LDS by
-// workgroups of 8 output crosslines, 2 crossline offsets per staged group, the
-// whole aperture in one launch (on the 401 x 401 x 1001 survey this beats the
-// gather form, which is bound by the texture-address rate); half-precision
-// traces are VALU-bound in either form and use the gather form.
+// workgroups of 8 output crosslines, 4 crossline offsets per staged group
+// (with both rows of a row pair summed into one window: 467 ms on the
+// 401 x 401 x 1001 survey against 517 ms with 2 offsets), the whole aperture
+// in one launch (this beats the gather form, which is bound by the
+// texture-address rate); half-precision traces are VALU-bound in either form
+// and use the gather form.
 constexpr int kAutoStageWaves = 8;
-constexpr int kAutoStageSteps = 2;
+constexpr int kAutoStageSteps = 4;
 
 // Bytes a row of the pair layout may span for the staged forms (the row is
 // addressed through a buffer resource with a 32-bit size, and the staging

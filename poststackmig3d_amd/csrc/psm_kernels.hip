@@ -699,6 +699,15 @@ struct StageTab {
 template <int ST, int W, int G, int MODE = kForward>
 struct LdsPlan {
   static constexpr bool kAdj = MODE == kAdjoint;
+  // FP32 forward: the rows +db and -db of a row pair share every step's tau,
+  // i, f, weight and mask, so a column's windows of the two rows are staged
+  // as one window of summed pairs (x_p[i] + x_m[i], d_p[i] + d_m[i]) -- one
+  // ring row, half the LDS reads and interpolations per step.  The adjoint
+  // likewise adds the same terms to both rows, so each wave accumulates one
+  // window per column and flushes it into both rows.  The per-trace scaled
+  // half-precision forward keeps one ring row per row.
+  static constexpr bool kSumRows = !kAdj && ST == PSM_STORE_F32;
+  static constexpr int kRingRows = (kSumRows || kAdj) ? 1 : 2;
   static constexpr int kB = kAdj ? 4 : Store<ST>::kBytes;  // bytes per pair / cell
   static constexpr int kSlot = kWinPairs * kB;           // bytes of one window
   static constexpr int kCols = kMaxCols + 4 * G + W;     // table entries (reads past nO stay inside)
@@ -708,19 +717,26 @@ struct LdsPlan {
   // group.  Single ring (G + W - 1 slots): they are stored after a barrier
   // that ends the current group's reads, two barriers per group; used when
   // the double ring would leave fewer than three workgroups per CU.
-  static constexpr bool kSingle = (4 * (2 * G + W - 1) + 1) * kSlot + kTables > 53 * 1024;
+  static constexpr bool kSingle =
+      (2 * kRingRows * (2 * G + W - 1) + 1) * kSlot + kTables > 53 * 1024;
   static constexpr int kRing = kSingle ? G + W - 1 : 2 * G + W - 1;
   static constexpr int kSide = kRing * kSlot;            // side 0 -> side 1
   static constexpr int kRow = 2 * kSide;                 // ring row 0 -> ring row 1
-  static constexpr int kSpare = 2 * kRow;                // one slot that absent jobs write
+  // one slot that absent staging jobs write; the adjoint's waves keep their
+  // windows in [0, W kSlot) instead of the ring
+  static constexpr int kSpare = kAdj ? W * kSlot : kRingRows * kRow;
   static constexpr int kMeta = kSpare + kSlot;           // int meta[2][kCols]
   static constexpr int kR2 = kMeta + 2 * kCols * 4;      // float r2tab[kCols]
   static constexpr int kPar = kR2 + kCols * 4;           // float par[5][W]
   static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
   static constexpr int kWid = kFlag + W * 4;             // adjoint: int wid[2][kCols]
   static constexpr int kClaim = kWid + (kAdj ? 2 * kCols * 4 : 0);  // adjoint: int claim[W][kWinPairs]
-  static constexpr int kLP = kClaim + (kAdj ? W * kWinPairs * 4 : 0);  // adjoint: float lp[5][W][kWave]
-  static constexpr int kBytes = kLP + (kAdj ? 5 * W * 64 * 4 : 0);
+  // adjoint: float4 lp[W][kWave] = (t0^2, 4/v^2, R^2, weight factor) of every
+  // output crossline's lanes
+  static constexpr int kLP = (kClaim + (kAdj ? W * kWinPairs * 4 : 0) + 15) / 16 * 16;
+  // adjoint: float dummy[W][kWave], the cell a lane adds to in a pass it sits out
+  static constexpr int kDummy = kLP + (kAdj ? W * 64 * 16 : 0);
+  static constexpr int kBytes = kDummy + (kAdj ? W * 64 * 4 : 0);
   // waves per SIMD the LDS allows (three 8-wave workgroups per CU with the
   // double ring), else no register target
   static constexpr int kWavesPerEu = (W == 8 && !kSingle && 3 * (kBytes + 16) <= 160 * 1024) ? 6 : 1;
@@ -821,26 +837,16 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
       p.t0sq = gTlo2;
       p.a = gAlo;
     }
-    // adjoint: every wave holds the parameters of all W output crosslines
-    // (its lanes' samples at each of them), published once through LDS
-    LaneParams pw[PL::kAdj ? W : 1];
+    // adjoint: the parameters of every output crossline's lanes, published
+    // once through LDS and read back per step: (t0^2, 4/v^2, R^2, weight
+    // factor), the factor being (1/v^2) m with amplitude correction (the
+    // weight times the image sample is then rsq(tau^2) times it) and m
+    // without
+    using F4 = float __attribute__((ext_vector_type(4)));
+    F4* const lp4 = reinterpret_cast<F4*>(smem + PL::kLP);
     if constexpr (PL::kAdj) {
-      float* const lp = reinterpret_cast<float*>(smem + PL::kLP);
-      lp[(0 * W + w) * kWave + lane] = p.t0sq;
-      lp[(1 * W + w) * kWave + lane] = p.a;
-      lp[(2 * W + w) * kWave + lane] = p.R2;
-      lp[(3 * W + w) * kWave + lane] = p.iv2;
-      lp[(4 * W + w) * kWave + lane] = p.mval;
+      lp4[w * kWave + lane] = F4{p.t0sq, p.a, p.R2, AMP ? p.iv2 * p.mval : p.mval};
       __syncthreads();
-#pragma unroll
-      for (int v = 0; v < W; ++v) {
-        pw[v] = p;
-        pw[v].t0sq = lp[(0 * W + v) * kWave + lane];
-        pw[v].a = lp[(1 * W + v) * kWave + lane];
-        pw[v].R2 = lp[(2 * W + v) * kWave + lane];
-        pw[v].iv2 = lp[(3 * W + v) * kWave + lane];
-        pw[v].mval = lp[(4 * W + v) * kWave + lane];
-      }
     }
 
     int dmax = (int)(__builtin_sqrtf(R2wg) / a.dx) + 2;
@@ -934,7 +940,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         if constexpr (PL::kAdj) {
           // the accumulators of both ring rows start at zero
           using Z4 = unsigned __attribute__((ext_vector_type(4)));
-          for (int off = (int)threadIdx.x * 16; off < 2 * PL::kRow; off += W * kWave * 16)
+          for (int off = (int)threadIdx.x * 16; off < W * PL::kSlot; off += W * kWave * 16)
             *reinterpret_cast<Z4*>(smem + off) = Z4{0u, 0u, 0u, 0u};
         }
         // (the vote runs with every lane active: lane l > 0 holds columns w +
@@ -952,47 +958,30 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           // v on side 0, o - (W - 1 - v) on side 1, the same (i, f, w, mask)
           // as the gather-form adjoint: d[i] += w (1 - f) m, d[i + 1] += w f m
           // -- into a window of its own (samples lo .. lo + kWinPairs - 1 of
-          // the column's trace in both ring rows; lo as in the table, hi - lo
-          // <= kWinPairs - 2), then flushes the window into the data cube
-          // (global atomic adds of its nonzero samples) and zeroes it.  No
-          // window is shared between waves, so the sweep needs no barrier.
-          // Within a wave, lanes with the same i collide: each round, the
-          // lanes still pending claim their i in the wave's claim row (one
-          // writer per address wins), the winners add (sample i, then i +
-          // 1), the others retry.
-          using V2 = float __attribute__((ext_vector_type(2)));
+          // the column's trace; lo as in the table, hi - lo <= kWinPairs - 2),
+          // then flushes the window into the data cube (global atomic adds of
+          // its nonzero samples) and zeroes it.  Rows +db and -db receive the
+          // same terms, so one window serves both and the flush adds it to
+          // each.  No window is shared between waves, so the sweep needs no
+          // barrier.
           const int* const wid = reinterpret_cast<const int*>(smem + PL::kWid);
           using LdsInt = volatile __attribute__((address_space(3))) int;  // LDS, not flat
           LdsInt* const claim = (LdsInt*)(smem + PL::kClaim) + w * kWinPairs;
-          const int wslot = w * PL::kSlot;  // this wave's window in ring row 0 (row 1: + kRow)
+          const int wslot = w * PL::kSlot;  // this wave's window
+          const bool odd = (lane & 1) != 0;
+          const int dmy = PL::kDummy + (w * kWave + lane) * 4;  // this lane's dummy cell
+          auto add_cell = [&](int ar, float cc) {
+            float* const p0 = reinterpret_cast<float*>(smem + ar);
+            *p0 = *p0 + cc;
+          };
+          // one pass: the lanes in m add cc to their cell, the others to their
+          // dummy cell (no branch, so no exec-mask bookkeeping between passes)
+          auto pass = [&](bool m, int ar, float cc) {
+            float* const p0 = reinterpret_cast<float*>(smem + (m ? ar : dmy));
+            *p0 = *p0 + cc;
+          };
           auto run_cols = [&](auto nr_c) {
             constexpr int NR = decltype(nr_c)::value;
-            auto rmw_rows = [&](int ar, float cc) {
-              float* const p0 = reinterpret_cast<float*>(smem + ar);
-              float* const p2 = reinterpret_cast<float*>(smem + ar + PL::kRow);
-              const float x0 = *p0;
-              const float x2 = NR == 2 ? *p2 : 0.0f;
-              *p0 = x0 + cc;
-              if constexpr (NR == 2) *p2 = x2 + cc;
-            };
-            // cells i (w0: add v0) and i + 1 (w1: add v1) of both rows, for
-            // lanes whose cells no other lane writes (any order of the reads
-            // and writes is then safe)
-            auto rmw_own = [&](int ar, bool w0, float v0, bool w1, float v1) {
-              if (!(w0 || w1)) return;
-              float* const p0 = reinterpret_cast<float*>(smem + ar);
-              float* const p2 = reinterpret_cast<float*>(smem + ar + PL::kRow);
-              const float x0 = p0[0], y0 = p0[1];
-              const float x2 = NR == 2 ? p2[0] : 0.0f, y2 = NR == 2 ? p2[1] : 0.0f;
-              if (w0) {
-                p0[0] = x0 + v0;
-                if constexpr (NR == 2) p2[0] = x2 + v0;
-              }
-              if (w1) {
-                p0[1] = y0 + v1;
-                if constexpr (NR == 2) p2[1] = y2 + v1;
-              }
-            };
             for (int job = w; job < 2 * nO; job += W) {
               const int side = job & 1, o = job >> 1;
               const int lo = uniform_i((PL::slot(side, o) - meta[side * PL::kCols + o]) / 4);
@@ -1002,45 +991,38 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                 const int dk = side == 0 ? o - v : o - (W - 1 - v);
                 if (dk < side || dk > kmax_wg) continue;
                 const float r2 = r2tab[dk];
-                const TauW tw = tau_weight(pw[v], r2, AMP);
+                const F4 pv = lp4[v * kWave + lane];
+                const LaneParams pr{pv.x, pv.y, pv.z, pv.w, 0, 0, 0.0f};
+                const TauW tw = tau_weight(pr, r2, AMP);
                 const float q = tw.tau * a.inv_dt;
                 const int i = (int)q;
-                const bool on = r2 <= pw[v].R2 && i < nt1;
+                const bool on = r2 <= pv.z && i < nt1;
                 const float f = __builtin_amdgcn_fractf(q);
-                const float wm = tw.w * pw[v].mval;
+                const float wm = AMP ? tw.w : pv.w;
                 const float c0 = wm - wm * f, c1 = wm * f;
                 int ar;
                 asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(ar) : "v"(i), "v"(bse));
                 // Fast path: i nondecreasing over the lanes with at most two
                 // lanes per value (the usual case: q advances by t0 / tau >=
-                // 1/2 per lane at apertures up to 60 deg).  A run of lanes
-                // with equal i sums its terms in its last lane, which alone
-                // writes cell i (the run's c0 plus the previous run's c1 when
-                // that run's i is i - 1) and cell i + 1 (the run's c1) unless
-                // the next run's i is i + 1 -- that run adds it instead.  A
-                // run writes only when it or the run below it has a term, so
-                // every cell written lies inside the window.
-                const int ip = lane_prev(i), ipp = lane_prev(ip), in_ = lane_next(i);
+                // 1/2 per lane at apertures up to 60 deg).  Then no two even
+                // lanes share an i, nor two odd lanes, and the terms go in
+                // four read-add-write passes whose cells are distinct within
+                // each pass: cell i of the even lanes, of the odd lanes, then
+                // cell i + 1 of the even and of the odd lanes.  The passes are
+                // ordered (a wave's LDS accesses complete in issue order; the
+                // empty asm statements keep the compiler from moving them).
+                const int ip = lane_prev(i), ipp = lane_prev(ip);
                 const bool dupp = lane > 0 && i == ip;
                 if (!__any((lane > 0 && i < ip) || (dupp && lane > 1 && ip == ipp))) {
-                  const bool dupn = lane < kWave - 1 && in_ == i;
-                  const float c0v = on ? c0 : 0.0f, c1v = on ? c1 : 0.0f;
-                  const float c0p = __int_as_float(lane_prev(__float_as_int(c0v)));
-                  const float c1p = __int_as_float(lane_prev(__float_as_int(c1v)));
-                  const int onp = lane_prev(on ? 1 : 0);
-                  // run sums and flag, complete in the run's last lane
-                  const float s0 = dupp ? c0v + c0p : c0v;
-                  const float s1 = dupp ? c1v + c1p : c1v;
-                  const int ron = (on || (dupp && onp != 0)) ? 1 : 0;
-                  // the previous run ends at lane l - 1, or l - 2 when lane
-                  // l - 1 belongs to this run
-                  const float s1p = __int_as_float(lane_prev(__float_as_int(s1)));
-                  const float s1pp = __int_as_float(lane_prev(__float_as_int(s1p)));
-                  const int ronp = lane_prev(ron), ronpp = lane_prev(ronp);
-                  const bool take = (dupp ? ronpp : ronp) != 0 && (dupp ? ipp : ip) + 1 == i;
-                  const bool last = !dupn;
-                  rmw_own(ar, last && (ron != 0 || take), take ? s0 + (dupp ? s1pp : s1p) : s0,
-                          last && ron != 0 && in_ != i + 1, s1);
+                  asm volatile("" ::: "memory");
+                  pass(on && !odd, ar, c0);
+                  asm volatile("" ::: "memory");
+                  pass(on && odd, ar, c0);
+                  asm volatile("" ::: "memory");
+                  pass(on && !odd, ar + 4, c1);
+                  asm volatile("" ::: "memory");
+                  pass(on && odd, ar + 4, c1);
+                  asm volatile("" ::: "memory");
                 } else {
                   // only lanes with a term take part (i of the others may lie
                   // outside the window)
@@ -1053,28 +1035,31 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                     // samples i first, then i + 1 (the winner at i + 1 adds
                     // there too): within each phase the winners' cells are
                     // distinct
-                    if (win) rmw_rows(ar, c0);
+                    if (win) add_cell(ar, c0);
                     asm volatile("" ::: "memory");
-                    if (win) rmw_rows(ar + 4, c1);
+                    if (win) add_cell(ar + 4, c1);
                     pending = pending && !win;
                   }
                 }
               }
               asm volatile("" ::: "memory");  // the window is complete
-              // flush: lane l holds the column's samples lo + 2l, lo + 2l + 1
+              // flush into each row: lane l holds the column's samples lo + l
+              // and lo + l + 64 (each atomic instruction covers 256
+              // contiguous bytes)
               const int k = side == 0 ? kbase + o : kbase + (W - 1) - o;
               const int n = wid[side * PL::kCols + o];
+              float* const cell = reinterpret_cast<float*>(smem + wslot);
+              const float v0 = cell[lane], v1 = cell[lane + kWave];
+              cell[lane] = 0.0f;
+              cell[lane + kWave] = 0.0f;
+              if (k >= xd_lo && k < xd_hi) {  // outside the data: nothing scattered there counts
 #pragma unroll
-              for (int rr = 0; rr < NR; ++rr) {
-                char* const cell = smem + wslot + rr * PL::kRow + lane * 8;
-                const V2 val = *reinterpret_cast<const V2*>(cell);
-                *reinterpret_cast<V2*>(cell) = V2{0.0f, 0.0f};
-                if (k >= xd_lo && k < xd_hi) {  // outside the data: nothing scattered there counts
+                for (int rr = 0; rr < NR; ++rr) {
                   float* const dst = a.adj_out +
                                      ((size_t)(rr == 0 ? r0 : rm) * a.n_xd + (k - xd_lo)) * a.n_t +
-                                     lo + 2 * lane;
-                  if (2 * lane < n && val.x != 0.0f) unsafeAtomicAdd(dst, val.x);
-                  if (2 * lane + 1 < n && val.y != 0.0f) unsafeAtomicAdd(dst + 1, val.y);
+                                     lo + lane;
+                  if (lane < n && v0 != 0.0f) unsafeAtomicAdd(dst, v0);
+                  if (lane + kWave < n && v1 != 0.0f) unsafeAtomicAdd(dst + kWave, v1);
                 }
               }
             }
@@ -1096,6 +1081,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // of range; columns outside the data are zeros (out of range in the
         // first fill, the row's zero trace afterwards).
         using V4 = unsigned __attribute__((ext_vector_type(4)));
+        using F4 = float __attribute__((ext_vector_type(4)));
         const int lb = lane * 16;
         const int lbs = lb < PL::kSlot ? lb : 0x7ff00000;  // lanes past the slot load nothing
         const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
@@ -1106,7 +1092,8 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // within the columns [oa, oa + nc).
         // (one return statement: a lambda returning this struct on two paths
         // made the compiler copy the kernel arguments to scratch)
-        const int jsh = two ? 2 : 1;  // log2 of the jobs per column
+        // log2 of the jobs per column (summed rows: one job loads both rows)
+        const int jsh = (two && !PL::kSumRows) ? 2 : 1;
         auto job_at = [&](int oa, int nc, int j) {
           return StageJob{oa + (j >> jsh), (j >> (jsh - 1)) & 1, j & (jsh - 1), j < (nc << jsh)};
         };
@@ -1123,7 +1110,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         auto tab = [&](int side, int o) { return meta[side * PL::kCols + o]; };
         {  // the first G + W - 1 columns, loaded straight into LDS (no
            // registers held); complete before the barrier
-          constexpr int kFill = (4 * (G + W - 1) + W - 1) / W;
+          constexpr int kFill = (2 * PL::kRingRows * (G + W - 1) + W - 1) / W;
           const int nc = min(G + W - 1, nO);
           int m[kFill];
 #pragma unroll
@@ -1137,10 +1124,19 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
             if (jb.ok && lb < PL::kSlot) {
               int voff, soff;
               job_src(jb, m[t], voff, soff);
-              __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                  jb.rr == 0 ? rs0 : rs1,
-                  (__attribute__((address_space(3))) void*)(smem + job_dst(jb)), 16, voff, soff,
-                  0, 0);
+              if (PL::kSumRows && two) {
+                // both rows through registers, their sum into the slot
+                const F4 x0 = __builtin_bit_cast(
+                    F4, __builtin_amdgcn_raw_buffer_load_b128(rs0, voff, soff, 0));
+                const F4 x1 = __builtin_bit_cast(
+                    F4, __builtin_amdgcn_raw_buffer_load_b128(rs1, voff, soff, 0));
+                *reinterpret_cast<F4*>(smem + job_dst(jb) + lb) = x0 + x1;
+              } else {
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    jb.rr == 0 ? rs0 : rs1,
+                    (__attribute__((address_space(3))) void*)(smem + job_dst(jb)), 16, voff, soff,
+                    0, 0);
+              }
             }
           }
           __builtin_amdgcn_s_waitcnt(0);
@@ -1152,7 +1148,8 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // latency overlaps the stores and the barrier: r^2, the two column
         // addresses of each step, and the table entries of the staging jobs
         // for the group after it.
-        constexpr int kJobs = (4 * G + W - 1) / W;  // staging jobs per wave per group, at most
+        // staging jobs per wave per group, at most
+        constexpr int kJobs = (2 * PL::kRingRows * G + W - 1) / W;
         // This wave's jobs in every full group of the row: job q is column
         // oa + jc of side js, ring row jr, if the wave has that job at all
         // (oa = the group's first new column), so the jobs' state advances
@@ -1290,9 +1287,11 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // of their reads: the loads issued at the start of group dk0 are held
         // in registers through groups dk0 and dk0 + G, stored at the end of
         // group dk0 + G and read from group dk0 + 2G on.
-        using Jobs = V4[kJobs];
+        // (summed rows: the second row's loads follow the first's, v[kJobs + q])
+        using Jobs = V4[PL::kSumRows ? 2 * kJobs : kJobs];
         using Dsts = int[kJobs];
-        auto issue = [&](const int* m, Jobs& v, Dsts& dst) {
+        auto issue = [&](auto nr_c, const int* m, Jobs& v, Dsts& dst) {
+          constexpr int NR = decltype(nr_c)::value;
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
             // a job loads pairs lo .. lo + kWinPairs - 1 of its column (lo * kB
@@ -1304,8 +1303,11 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
             const bool data = (unsigned)jkd[q] < (unsigned)a.n_xd;
             dst[q] = jo[q] < jlim[q] ? jsl[q] : PL::kSpare;
             const int voff = jsl[q] + jlbs[q] - m[q];  // lo * kB + lane * 16
-            v[q] = __builtin_amdgcn_raw_buffer_load_b128(jrs[q], voff > nt1 * kB ? 0x7ff00000 : voff,
-                                                         data ? jso[q] : a.n_xd * tb, 0);
+            const int vo = voff > nt1 * kB ? 0x7ff00000 : voff;
+            const int so = data ? jso[q] : a.n_xd * tb;
+            v[q] = __builtin_amdgcn_raw_buffer_load_b128(jrs[q], vo, so, 0);
+            if constexpr (PL::kSumRows && NR == 2)
+              v[PL::kSumRows ? kJobs + q : q] = __builtin_amdgcn_raw_buffer_load_b128(rs1, vo, so, 0);
             jo[q] += G;
             jkd[q] += jdkd[q];
             jso[q] += jdkd[q] * tb;
@@ -1322,9 +1324,12 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           // stores and barrier -- is not held up behind other waves'
           // sums; 0.75 % on the production survey)
           __builtin_amdgcn_s_setprio(1);
-          issue(t.mj, vi, di);
+          issue(nr_c, t.mj, vi, di);
           __builtin_amdgcn_s_setprio(0);
-          compute(nr_c, full_c, first_c, mask_c, dk0, min(G, kmax_wg + 1 - dk0), t);
+          // (summed rows: one ring row holds both rows of the pair)
+          constexpr int NR = decltype(nr_c)::value;
+          compute(std::integral_constant<int, PL::kSumRows ? 1 : NR>{}, full_c, first_c, mask_c,
+                  dk0, min(G, kmax_wg + 1 - dk0), t);
           read_tab(dk0 + G, t);
           // The stores wait on the loads, so the group's sum must be complete
           // before them: the empty asm takes the accumulator as an input, and
@@ -1335,8 +1340,16 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           __builtin_amdgcn_sched_barrier(0);
           if constexpr (PL::kSingle) __syncthreads();  // this group's reads are done
 #pragma unroll
-          for (int q = 0; q < kJobs; ++q)
-            if (lb < PL::kSlot) *reinterpret_cast<V4*>(smem + ds[q] + lb) = vs[q];
+          for (int q = 0; q < kJobs; ++q) {
+            if (lb < PL::kSlot) {
+              if constexpr (PL::kSumRows && NR == 2)
+                *reinterpret_cast<F4*>(smem + ds[q] + lb) =
+                    __builtin_bit_cast(F4, vs[q]) +
+                    __builtin_bit_cast(F4, vs[PL::kSumRows ? kJobs + q : q]);
+              else
+                *reinterpret_cast<V4*>(smem + ds[q] + lb) = vs[q];
+            }
+          }
           __syncthreads();  // the next group's windows are visible
           __builtin_amdgcn_s_setprio(0);
         };
@@ -1349,7 +1362,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
             int m0[kJobs];
 #pragma unroll
             for (int q = 0; q < kJobs; ++q) m0[q] = tab(js[q], jo[q]);
-            issue(m0, va, da);
+            issue(nr_c, m0, va, da);
           }
           Tab t;
           read_tab(0, t);
@@ -1424,11 +1437,8 @@ void migrate_lds_kernel(MigArgs a, int n_tiles, int n_jgroups, int per_tile_pad)
 }
 
 // The staged adjoint over the same grid as migrate_lds_kernel (FP32 data).
-// (8-wave workgroups: registers capped at 80 so that three of them share a
-// CU, as the LDS allows; 89 registers left room for two, 3 % slower)
 template <int W, int G, bool AMP>
 __global__ __launch_bounds__(kWave * W)
-__attribute__((amdgpu_waves_per_eu(W == 8 ? 6 : 1)))
 void demigrate_lds_kernel(MigArgs a, int n_tiles, int n_jgroups, int per_tile_pad) {
   __shared__ __attribute__((aligned(16))) char smem[LdsPlan<PSM_STORE_F32, W, G, kAdjoint>::kBytes];
   const int g = blockIdx.x;
@@ -1717,11 +1727,11 @@ hipError_t launch_lds_wg(const MigArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-// staged adjoint: 8 output crosslines per workgroup (each column window
-// collects the terms of all of them before its one flush; on the 201 x 201 x
-// 751 least-squares cube 68.6 ms, against 96.4 ms with 4 and 163 ms with 2)
+// staged adjoint: kAdjWaves output crosslines per workgroup (each column
+// window collects the terms of all of them before its one flush)
+constexpr int kAdjWaves = 8;
 hipError_t launch_adj_lds(const MigArgs& a, hipStream_t stream) {
-  constexpr int W = 8, G = 1;
+  constexpr int W = kAdjWaves, G = 1;
   const int n_tiles = (a.n_t + kWave - 1) / kWave;
   const int n_jgroups = (a.n_out_xl + W - 1) / W;
   const long long per_tile = (long long)a.n_out_il * n_jgroups;

@@ -42,9 +42,9 @@ def _reference(items, storage, wpb, staged):
     try:
         ctx.set_storage(storage)
         if staged:
-            # the batch's staged form: 8 crosslines, 2 offsets per group, one launch
+            # the batch's staged form: 8 crosslines, 4 offsets per group, one launch
             ctx.set_tuning(8, 0)
-            ctx.set_staging(True, 2)
+            ctx.set_staging(True, 4)
         else:
             ctx.set_tuning(wpb, 0)
             ctx.set_staging(False)

@@ -222,7 +222,7 @@ def test_volume_launch_shapes(wpb, rows):
             ctx.set_staging(staged, 2)
             got = P.migrate_volume(data, vel, prm, ctx=ctx)
             assert rel_l2(got, ref) <= TOL, staged
-        ctx.set_staging(None, 2)
+        ctx.set_staging(None, 4)
         # the per-inline drop-in under the same shape
         vmax = O.estimate_max_velocity(vel)
         tan2, ap = O.aperture(45.0, vmax, n[2], 0.002, 25.0)

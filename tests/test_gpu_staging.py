@@ -97,7 +97,7 @@ def _migrate(ctx, data, vel, prm, staging, storage="f32", wpb=4, steps=4, rows=-
     finally:
         ctx.set_storage("f32")
         ctx.set_tuning(4, -1)
-        ctx.set_staging(None, 2)
+        ctx.set_staging(None, 4)
 
 
 @pytest.mark.parametrize("name", CASE_NAMES)
@@ -135,7 +135,7 @@ def test_staged_adjoint_matches_gather_adjoint(ctx, name):
     try:
         gather = P.demigrate_volume(image, vel, prm, ctx=ctx)
     finally:
-        ctx.set_staging(None, 2)
+        ctx.set_staging(None, 4)
     staged = P.demigrate_volume(image, vel, prm, ctx=ctx)
     assert np.isfinite(staged).all()
     e = rel_l2(staged, gather)
@@ -165,25 +165,25 @@ def test_staged_row_passes(ctx):
 
 
 def test_staging_switch_roundtrip(ctx):
-    assert ctx.get_staging() == (None, 2)  # default: chosen by trace storage
+    assert ctx.get_staging() == (None, 4)  # default: chosen by trace storage
     ctx.set_staging(False, 2)
     assert ctx.get_staging() == (False, 2)
     ctx.set_staging(True, 4)
     assert ctx.get_staging() == (True, 4)
     with pytest.raises(PsmError):
         ctx.set_staging(True, 3)
-    ctx.set_staging(None, 2)
+    ctx.set_staging(None, 4)
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 def test_automatic_form_matches_forced_forms(ctx, storage):
     """The default (automatic) form equals the form it selects: staged with 8
-    crosslines per workgroup and 2 offsets per group for FP32 traces, gather
+    crosslines per workgroup and 4 offsets per group for FP32 traces, gather
     for half-precision traces."""
     data, vel, prm, _ = _case("padded_lateral")
     auto = _migrate(ctx, data, vel, prm, staging=None, storage=storage)
     if storage == "f32":
-        forced = _migrate(ctx, data, vel, prm, staging=True, storage=storage, wpb=8, steps=2,
+        forced = _migrate(ctx, data, vel, prm, staging=True, storage=storage, wpb=8, steps=4,
                           rows=0)
     else:
         forced = _migrate(ctx, data, vel, prm, staging=False, storage=storage)
