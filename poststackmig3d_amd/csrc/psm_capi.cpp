@@ -833,7 +833,7 @@ struct psm_batch {
   int n_cu = 0;
   std::vector<psm::MigArgs> args;
   std::vector<psm::BatchSeg> segs;
-  unsigned total = 0;
+  unsigned totals[psm::kBatchXcds] = {};  // items per XCD queue
   // per distinct data cube: raw pointer, shape, packed buffer, scales
   struct Group {
     const float* data;
@@ -842,7 +842,7 @@ struct psm_batch {
   };
   std::vector<std::unique_ptr<Group>> groups;
   std::vector<std::unique_ptr<DevBuf>> item_bufs;  // tasks and t0 per item
-  DevBuf d_args, d_segs, counter;
+  DevBuf d_args, d_segs, d_totals, counter;
   std::vector<int> n_tiles;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -867,7 +867,7 @@ int batch_capture(psm_batch* b) {
                                 : b->staged     ? kBatchStagedPerCu
                                                 : std::max(1, 32 / b->wpb));
   PSM_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  hipError_t e = psm::launch_fill_zero(b->counter.as<float>(), 1, st);
+  hipError_t e = psm::launch_fill_zero(b->counter.as<float>(), psm::kBatchXcds, st);
   for (auto& g : b->groups) {
     if (e != hipSuccess) break;
     e = psm::launch_pack(b->storage, g->data, g->pairs.p, g->scale.as<float>(), g->n_il, g->n_xl,
@@ -875,8 +875,8 @@ int batch_capture(psm_batch* b) {
   }
   if (e == hipSuccess)
     e = psm::launch_batch(b->storage, b->staged, b->d_args.as<psm::MigArgs>(),
-                          b->d_segs.as<psm::BatchSeg>(),
-                          (int)b->segs.size(), b->counter.as<unsigned>(), b->total, b->wpb, blocks,
+                          b->d_segs.as<psm::BatchSeg>(), (int)b->segs.size(),
+                          b->counter.as<unsigned>(), b->d_totals.as<unsigned>(), b->wpb, blocks,
                           st);
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(st, &g);
@@ -911,21 +911,36 @@ int batch_set_shape(psm_batch* b, int wpb, int blocks_per_cu) {
   b->segs.clear();
   int max_tiles = 0;
   for (int t : b->n_tiles) max_tiles = std::max(max_tiles, t);
-  long long first = 0;
+  // segments in queue order (latest t0 tiles of all volumes first); each
+  // segment's items split into kBatchXcds contiguous chunks, chunk x appended
+  // to XCD x's queue
+  long long first[psm::kBatchXcds] = {};
   for (int r = 0; r < max_tiles; ++r)
     for (int k = 0; k < b->n_items; ++k) {
       if (b->n_tiles[k] <= r) continue;
       const psm::MigArgs& a = b->args[k];
       const int njg = (a.n_out_xl + wpb - 1) / wpb;
       const long long cnt = (long long)a.n_out_il * njg;
-      if (first + cnt > 0x7fffffffLL) return fail(c, PSM_ERR_ARG, "batch queue too long");
-      b->segs.push_back(psm::BatchSeg{k, b->n_tiles[k] - 1 - r, (int)first, (int)cnt, njg});
-      first += cnt;
+      if (cnt > 0x7fffffffLL) return fail(c, PSM_ERR_ARG, "batch queue too long");
+      psm::BatchSeg sg{};
+      sg.item = k;
+      sg.tile = b->n_tiles[k] - 1 - r;
+      sg.count = (int)cnt;
+      sg.n_jgroups = njg;
+      for (int x = 0; x < psm::kBatchXcds; ++x) {
+        const long long lo = cnt * x / psm::kBatchXcds, hi = cnt * (x + 1) / psm::kBatchXcds;
+        if (first[x] + (hi - lo) > 0x7fffffffLL) return fail(c, PSM_ERR_ARG, "batch queue too long");
+        sg.xbase[x] = (int)lo;
+        sg.xfirst[x] = (int)first[x];
+        first[x] += hi - lo;
+      }
+      b->segs.push_back(sg);
     }
-  b->total = (unsigned)first;
+  for (int x = 0; x < psm::kBatchXcds; ++x) b->totals[x] = (unsigned)first[x];
   hipStream_t st = c->own_stream;
   PSM_HIP(c, upload(b->d_args, b->args.data(), b->args.size() * sizeof(psm::MigArgs), st));
   PSM_HIP(c, upload(b->d_segs, b->segs.data(), b->segs.size() * sizeof(psm::BatchSeg), st));
+  PSM_HIP(c, upload(b->d_totals, b->totals, sizeof b->totals, st));
   PSM_HIP(c, hipStreamSynchronize(st));
   return batch_capture(b);
 }
@@ -1116,6 +1131,7 @@ void psm_batch_destroy(psm_batch* b) {
   for (auto& d : b->item_bufs) d->release();
   b->d_args.release();
   b->d_segs.release();
+  b->d_totals.release();
   b->counter.release();
   delete b;
 }

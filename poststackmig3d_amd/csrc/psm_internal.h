@@ -56,10 +56,21 @@ struct MigArgs {
   float* adj_out;          // adjoint: data [n_rows][n_xd][n_t], accumulated (zeroed by the caller)
 };
 
+// The batch work queue is split by XCD (workgroups are dispatched to the 8
+// XCDs round robin by workgroup index): XCD x works through its own queue
+// first, then helps the others, so the items each XCD runs at a time are
+// neighbours that share aperture rows in its L2.
+constexpr int kBatchXcds = 8;
+
 // One segment of a batch's work queue: the crossline groups of one t0 tile of
-// one volume, occupying queue positions [first, first + count).
+// one volume (`count` items, spatial index io * n_jgroups + group).  Its items
+// are split into kBatchXcds contiguous chunks; chunk x (spatial indices from
+// xbase[x]) occupies positions [xfirst[x], xfirst[x + 1 segment]) of XCD x's
+// queue.
 struct BatchSeg {
-  int item, tile, first, count, n_jgroups;
+  int item, tile, count, n_jgroups;
+  int xbase[kBatchXcds];
+  int xfirst[kBatchXcds];
 };
 
 // Launch-shape parameters (chosen by the autotuner or the caller).
@@ -117,13 +128,15 @@ hipError_t launch_table_velocity(const psm_velocity_point* pts, int n_pts, float
 
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream);
 // Persistent batch launch (forward migration of several volumes): `blocks`
-// workgroups of wpb waves drain `total` queue positions described by segs.
-// staged: workgroups of the staged form (kAutoStageWaves output crosslines,
-// kAutoStageSteps offsets per group; items of that width, wpb unused), else
-// the gather form with wpb waves per workgroup.
+// workgroups of wpb waves drain the kBatchXcds queues described by segs;
+// counters[x] (zero at launch) counts the positions taken from XCD x's queue,
+// totals[x] its length (both device memory).  staged: workgroups of the
+// staged form (kAutoStageWaves output crosslines, kAutoStageSteps offsets per
+// group; items of that width, wpb unused), else the gather form with wpb
+// waves per workgroup.
 hipError_t launch_batch(int storage, bool staged, const MigArgs* items, const BatchSeg* segs,
-                        int n_segs, unsigned* counter, unsigned total, int wpb, int blocks,
-                        hipStream_t stream);
+                        int n_segs, unsigned* counters, const unsigned* totals, int wpb,
+                        int blocks, hipStream_t stream);
 // out[0] = sum x[i] y[i] in float64; partial holds dot_scratch_doubles() - 1 doubles.
 size_t dot_scratch_doubles();
 hipError_t launch_dot_f64(const float* x, const float* y, size_t n, double* partial, double* out,

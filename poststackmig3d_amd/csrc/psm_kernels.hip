@@ -44,6 +44,7 @@
 // -ffp-contract=off so that these FMAs are exactly the ones written here.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <type_traits>
 
 #include "psm_internal.h"
@@ -980,6 +981,13 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
             float* const p0 = reinterpret_cast<float*>(smem + (m ? ar : dmy));
             *p0 = *p0 + cc;
           };
+          // With one 4/v^2 over the workgroup's tile (laterally and
+          // temporally constant velocity) q = sqrt(t0^2 + a r^2) / dt grows by
+          // t0 / tau >= cos(theta) per lane inside the aperture, so at
+          // apertures below about 60 deg (cos > 1/2 with a margin) the fast
+          // path's condition holds for every lane with a term: no check.
+          const bool mono = gAlo == gAhi && a.alpha_mask * a.alpha_mask * a.tan2 <
+                                                2.9f * (a.alpha * a.alpha);
           auto run_cols = [&](auto nr_c) {
             constexpr int NR = decltype(nr_c)::value;
             for (int job = w; job < 2 * nO; job += W) {
@@ -1011,9 +1019,13 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                 // cell i + 1 of the even and of the odd lanes.  The passes are
                 // ordered (a wave's LDS accesses complete in issue order; the
                 // empty asm statements keep the compiler from moving them).
-                const int ip = lane_prev(i), ipp = lane_prev(ip);
-                const bool dupp = lane > 0 && i == ip;
-                if (!__any((lane > 0 && i < ip) || (dupp && lane > 1 && ip == ipp))) {
+                bool fast = mono;
+                if (!mono) {
+                  const int ip = lane_prev(i), ipp = lane_prev(ip);
+                  const bool dupp = lane > 0 && i == ip;
+                  fast = !__any((lane > 0 && i < ip) || (dupp && lane > 1 && ip == ipp));
+                }
+                if (fast) {
                   asm volatile("" ::: "memory");
                   pass(on && !odd, ar, c0);
                   asm volatile("" ::: "memory");
@@ -1451,42 +1463,72 @@ void demigrate_lds_kernel(MigArgs a, int n_tiles, int n_jgroups, int per_tile_pa
                                                  (spatial % n_jgroups) * W, smem);
 }
 
+// One dword of a read-only table through the constant address space (a
+// scalar load at a wave-uniform index).
+template <class T>
+__device__ __forceinline__ int const_dword(const T* base, int idx) {
+  using CI = const __attribute__((address_space(4))) int*;
+  return ((CI)base)[idx];
+}
+
+// The next work item of a persistent batch workgroup on XCD `xcd`: from that
+// XCD's queue first, then from the other XCDs' queues in turn (`pass` counts
+// the queues found drained).  Returns false once every queue is drained.  All
+// waves of the workgroup call it together; s_next is an LDS word.  The queue
+// position comes from LDS and is made uniform explicitly, so everything
+// derived from it -- the segment, the item's arguments and buffer resources
+// -- is read through scalar loads and stays in scalar registers.
+__device__ __forceinline__ bool batch_next(const BatchSeg* segs, int n_segs, unsigned* counters,
+                                           const unsigned* totals, int xcd, int& pass,
+                                           unsigned* s_next, BatchSeg& sg, int& spatial) {
+  constexpr int kXfirst = (int)(offsetof(BatchSeg, xfirst) / 4);
+  for (; pass < kBatchXcds; ++pass) {
+    const int x = uniform_i((xcd + pass) % kBatchXcds);
+    const unsigned total = (unsigned)const_dword(totals, x);
+    if (threadIdx.x == 0) *s_next = total ? atomicAdd(&counters[x], 1u) : 0u;
+    __syncthreads();
+    const unsigned wi = (unsigned)uniform_i((int)*s_next);
+    __syncthreads();
+    if (wi >= total) continue;
+    int lo = 0, hi = n_segs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      const int f = const_dword(segs + mid, kXfirst + x);
+      if ((unsigned)f <= wi) lo = mid; else hi = mid - 1;
+    }
+    sg = const_load<BatchSeg>(segs + lo);
+    spatial = const_dword(segs + lo, (int)(offsetof(BatchSeg, xbase) / 4) + x) +
+              (int)(wi - (unsigned)const_dword(segs + lo, kXfirst + x));
+    return true;
+  }
+  return false;
+}
+
 // Persistent batch kernel: every workgroup takes work items (one
-// (volume, t0 tile, crossline group) each) from a global counter until the
-// queue is drained, so volumes of very different size and cost share one
-// launch.  Items are ordered by segment: segment k = the groups of one t0
-// tile of one volume, latest tiles of all volumes first.  Each workgroup
-// exits once the counter passes `total`.
+// (volume, t0 tile, crossline group) each) from the queues until all are
+// drained, so volumes of very different size and cost share one launch.
+// Each queue is ordered by segment: segment k = the groups of one t0 tile of
+// one volume, latest tiles of all volumes first.
 template <int ST, int WPB>
 __global__ __launch_bounds__(kWave * WPB) void batch_kernel(const MigArgs* __restrict__ items,
                                                             const BatchSeg* __restrict__ segs,
                                                             int n_segs,
-                                                            unsigned* __restrict__ counter,
-                                                            unsigned total) {
+                                                            unsigned* __restrict__ counters,
+                                                            const unsigned* __restrict__ totals) {
   __shared__ unsigned s_next;
-  for (;;) {
-    if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
-    __syncthreads();
-    // (read from LDS, so made uniform explicitly: everything derived from it
-    // -- the item's arguments, its buffer resources -- stays scalar)
-    const unsigned w = (unsigned)uniform_i((int)s_next);
-    __syncthreads();
-    if (w >= total) return;
-    int lo = 0, hi = n_segs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if ((unsigned)segs[mid].first <= w) lo = mid; else hi = mid - 1;
-    }
-    const BatchSeg sg = segs[lo];
-    const int spatial = (int)w - sg.first;
-    const MigArgs& a = items[sg.item];
+  const int xcd = (int)(blockIdx.x % kBatchXcds);
+  int pass = 0;
+  BatchSeg sg;
+  int spatial;
+  while (batch_next(segs, n_segs, counters, totals, xcd, pass, &s_next, sg, spatial)) {
+    const MigArgs a = const_load<MigArgs>(items + sg.item);
     const int io = spatial / sg.n_jgroups;
     const int j = (spatial % sg.n_jgroups) * WPB + uniform_i((int)(threadIdx.x / kWave));
     if (io < a.n_out_il && j < a.n_out_xl) migrate_trace_tile<ST, false, kForward>(a, sg.tile, io, j);
   }
 }
 
-// The same queue over workgroups of the staged form (W output crosslines per
+// The same queues over workgroups of the staged form (W output crosslines per
 // item): each item runs staged_tile, so its result equals migrate_lds_kernel's
 // for that (tile, inline, crossline group) bit for bit.  The barriers around
 // the queue read also separate one item's LDS use from the next.
@@ -1496,24 +1538,15 @@ template <int ST, int W, int G>
 __global__ __launch_bounds__(kWave * W)
 __attribute__((amdgpu_waves_per_eu(LdsPlan<ST, W, G>::kWavesPerEu)))
 void batch_lds_kernel(const MigArgs* __restrict__ items, const BatchSeg* __restrict__ segs,
-                      int n_segs, unsigned* __restrict__ counter, unsigned total) {
+                      int n_segs, unsigned* __restrict__ counters,
+                      const unsigned* __restrict__ totals) {
   __shared__ __attribute__((aligned(16))) char smem[LdsPlan<ST, W, G>::kBytes];
   __shared__ unsigned s_next;
-  for (;;) {
-    if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
-    __syncthreads();
-    const unsigned wi = (unsigned)uniform_i((int)s_next);  // uniform (see batch_kernel)
-    __syncthreads();
-    if (wi >= total) return;
-    // the queue and the item's arguments through the constant address space:
-    // scalar loads, so the item's buffer resources stay in scalar registers
-    int lo = 0, hi = n_segs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if ((unsigned)const_load<BatchSeg>(segs + mid).first <= wi) lo = mid; else hi = mid - 1;
-    }
-    const BatchSeg sg = const_load<BatchSeg>(segs + lo);
-    const int spatial = (int)wi - sg.first;
+  const int xcd = (int)(blockIdx.x % kBatchXcds);
+  int pass = 0;
+  BatchSeg sg;
+  int spatial;
+  while (batch_next(segs, n_segs, counters, totals, xcd, pass, &s_next, sg, spatial)) {
     const MigArgs a = const_load<MigArgs>(items + sg.item);
     const int io = spatial / sg.n_jgroups;
     const int j0 = (spatial % sg.n_jgroups) * W;
@@ -1829,31 +1862,31 @@ size_t dot_scratch_doubles() { return kDotBlocks + 1; }
 
 template <int ST>
 hipError_t launch_batch_st(bool staged, const MigArgs* items, const BatchSeg* segs, int n_segs,
-                           unsigned* counter, unsigned total, int wpb, int blocks,
+                           unsigned* counters, const unsigned* totals, int wpb, int blocks,
                            hipStream_t stream) {
   const dim3 grid((unsigned)blocks);
   if (staged) {
     hipLaunchKernelGGL((batch_lds_kernel<ST, kAutoStageWaves, kAutoStageSteps>), grid,
-                       dim3(kWave * kAutoStageWaves), 0, stream, items, segs, n_segs, counter,
-                       total);
+                       dim3(kWave * kAutoStageWaves), 0, stream, items, segs, n_segs, counters,
+                       totals);
     return hipGetLastError();
   }
   switch (wpb) {
-    case 1: hipLaunchKernelGGL((batch_kernel<ST, 1>), grid, dim3(kWave), 0, stream, items, segs, n_segs, counter, total); break;
-    case 2: hipLaunchKernelGGL((batch_kernel<ST, 2>), grid, dim3(2 * kWave), 0, stream, items, segs, n_segs, counter, total); break;
-    case 8: hipLaunchKernelGGL((batch_kernel<ST, 8>), grid, dim3(8 * kWave), 0, stream, items, segs, n_segs, counter, total); break;
-    default: hipLaunchKernelGGL((batch_kernel<ST, 4>), grid, dim3(4 * kWave), 0, stream, items, segs, n_segs, counter, total); break;
+    case 1: hipLaunchKernelGGL((batch_kernel<ST, 1>), grid, dim3(kWave), 0, stream, items, segs, n_segs, counters, totals); break;
+    case 2: hipLaunchKernelGGL((batch_kernel<ST, 2>), grid, dim3(2 * kWave), 0, stream, items, segs, n_segs, counters, totals); break;
+    case 8: hipLaunchKernelGGL((batch_kernel<ST, 8>), grid, dim3(8 * kWave), 0, stream, items, segs, n_segs, counters, totals); break;
+    default: hipLaunchKernelGGL((batch_kernel<ST, 4>), grid, dim3(4 * kWave), 0, stream, items, segs, n_segs, counters, totals); break;
   }
   return hipGetLastError();
 }
 
 hipError_t launch_batch(int storage, bool staged, const MigArgs* items, const BatchSeg* segs,
-                        int n_segs, unsigned* counter, unsigned total, int wpb, int blocks,
-                        hipStream_t stream) {
+                        int n_segs, unsigned* counters, const unsigned* totals, int wpb,
+                        int blocks, hipStream_t stream) {
   switch (storage) {
-    case PSM_STORE_F16: return launch_batch_st<PSM_STORE_F16>(staged, items, segs, n_segs, counter, total, wpb, blocks, stream);
-    case PSM_STORE_BF16: return launch_batch_st<PSM_STORE_BF16>(staged, items, segs, n_segs, counter, total, wpb, blocks, stream);
-    default: return launch_batch_st<PSM_STORE_F32>(staged, items, segs, n_segs, counter, total, wpb, blocks, stream);
+    case PSM_STORE_F16: return launch_batch_st<PSM_STORE_F16>(staged, items, segs, n_segs, counters, totals, wpb, blocks, stream);
+    case PSM_STORE_BF16: return launch_batch_st<PSM_STORE_BF16>(staged, items, segs, n_segs, counters, totals, wpb, blocks, stream);
+    default: return launch_batch_st<PSM_STORE_F32>(staged, items, segs, n_segs, counters, totals, wpb, blocks, stream);
   }
 }
 hipError_t launch_table_velocity(const psm_velocity_point* pts, int n_pts, float vmin,
