@@ -116,6 +116,9 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="output inlines in the CPU sample (0: 4 per host thread)")
     ap.add_argument("--cpu-xl", type=int, default=32, help="output crosslines per sampled inline")
+    ap.add_argument("--counters-out", default=None,
+                    help="also write the dominant kernel's counters (raw sums and derived "
+                         "figures, stamped with the kernel source sha) to this JSON file")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--counter-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
@@ -772,6 +775,21 @@ def main():
             derived = derive_counters(raw, avg_kernel_s)
             source = ("live: rocprofv3 --pmc child runs of this script (one migration step each, "
                       f"kernel sha {kernel_source_sha()})")
+            if args.counters_out:
+                rec = {}
+                if os.path.exists(args.counters_out):
+                    with open(args.counters_out) as f:
+                        rec = json.load(f)
+                if rec.get("kernel_sha16") != kernel_source_sha():
+                    rec = {}
+                rec.update({"kernel_sha16": kernel_source_sha(), "workload": args.workload,
+                            "fetch_scale": FETCH_SCALE, "write_scale": WRITE_SCALE,
+                            "counter_groups": COUNTER_GROUPS})
+                rec[args.storage] = dict(derived, form=form, kernel_ms=round(avg_kernel_s * 1e3, 3),
+                                         raw=raw, p_ap=p_ap, p_g=p_g,
+                                         valu_insts_per_pair=derived["valu_insts"] / p_ap)
+                with open(args.counters_out, "w") as f:
+                    json.dump(rec, f, indent=1)
     if derived is None:
         ent, source = committed_counters(args.storage, form)
         if ent:
