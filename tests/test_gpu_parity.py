@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 import pyoracle as O
-from _cases import reference_inline, reference_volume, rel_l2, t0_table, window_call
+from _cases import (TRACE_TOL, reference_inline, reference_volume, rel_l2, t0_table,
+                    window_call, worst_trace_rel)
 
 import poststackmig3d_amd as P
 from poststackmig3d_amd import synth
@@ -136,6 +137,9 @@ def test_volume_point_diffractor_101(gpu_ctx):
     got = P.migrate_volume(data, vel, prm)
     err = rel_l2(got, ref)
     assert err <= TOL, err
+    worst = worst_trace_rel(got, ref)
+    print(f"101x101x501: rel-L2 {err:.2e}, worst trace {worst:.2e}")
+    assert worst <= TRACE_TOL, worst
     # focusing at the apex (il 50, xl 50, 0.5 s)
     apex = np.unravel_index(np.argmax(np.abs(got)), got.shape)
     assert apex[0] == 50 and apex[1] == 50 and abs(apex[2] - 250) <= 2
@@ -152,6 +156,9 @@ def test_volume_padded_lateral_velocity(gpu_ctx):
     got = P.migrate_volume(data, vel, prm)
     assert got.shape == ref.shape == (41 + 12, 37 + 10, 201)
     assert rel_l2(got, ref) <= TOL
+    worst = worst_trace_rel(got, ref)
+    print(f"padded lateral: worst trace {worst:.2e}")
+    assert worst <= TRACE_TOL, worst
     # padded output inlines duplicate the edge inline (main.cpp:268-285)
     assert np.array_equal(got[0], got[6]) and np.array_equal(got[-1], got[-7])
     assert not got[:, :5].any()

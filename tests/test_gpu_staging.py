@@ -13,7 +13,7 @@ both storage forms.
 import numpy as np
 import pytest
 
-from _cases import reference_volume, rel_l2
+from _cases import TRACE_TOL, reference_volume, rel_l2, worst_trace_rel
 
 import poststackmig3d_amd as P
 from poststackmig3d_amd import synth
@@ -112,9 +112,11 @@ def test_staged_matches_reference_and_gather(ctx, name):
             got = _migrate(ctx, data, vel, prm, staging=True, wpb=wpb, steps=steps)
             assert np.isfinite(got).all()
             e_ref, e_g = rel_l2(got, ref), rel_l2(got, gather)
-            print(f"  wpb {wpb} steps {steps}: vs reference {e_ref:.2e}, vs gather {e_g:.2e}",
-                  flush=True)
+            worst = worst_trace_rel(got, ref)
+            print(f"  wpb {wpb} steps {steps}: vs reference {e_ref:.2e} (worst trace "
+                  f"{worst:.2e}), vs gather {e_g:.2e}", flush=True)
             assert e_ref <= TOL_REF, (wpb, steps, e_ref)
+            assert worst <= TRACE_TOL, (wpb, steps, worst)
             assert e_g <= TOL_FORMS, (wpb, steps, e_g)
 
 
