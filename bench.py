@@ -407,7 +407,9 @@ def run_scan(args, torch, P, dev):
     for storage in ("f32", "f16"):
         b = Batch(items, storage=storage, ctx=ctx)
         w, bpc, best = b.autotune()
-        log(f"scan {storage}: autotuned waves/block={w} blocks/CU={bpc} ({best:.1f} ms)")
+        staged_form, _, _, g = b.shape()
+        log(f"scan {storage}: autotuned waves/block={w} blocks/CU={bpc} "
+            f"{'staged, %d offsets per group' % g if staged_form else 'gather form'} ({best:.1f} ms)")
         for _ in range(args.warmup):
             b.run()
         ms = []
@@ -420,7 +422,8 @@ def run_scan(args, torch, P, dev):
         wall = time.perf_counter() - t_wall
         res[storage] = {"value": p_ap / (wall / args.steps), "ms_per_step": wall * 1e3 / args.steps,
                         "graph_ms": sum(ms) / len(ms), "waves_per_block": w,
-                        "blocks_per_cu": bpc}
+                        "blocks_per_cu": bpc, "form": "staged" if staged_form else "gather",
+                        "offsets_per_group": g}
         b.close()
     f32 = res["f32"]
     line = {

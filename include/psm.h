@@ -199,9 +199,11 @@ int psm_migrate_volume_dalpha(psm_ctx* ctx, const psm_survey* survey,
 /*
  * Batched migration (no reference counterpart): several volumes -- e.g. a
  * velocity scan over scaled Vrms models and sub-volumes of any size -- in one
- * persistent launch.  Every workgroup drains a shared queue of
- * (volume, t0 tile, crossline group) items, latest tiles of all volumes
- * first, so uneven volumes balance.  psm_batch_create packs nothing yet: it
+ * persistent launch.  Workgroups drain queues of (volume, t0 tile,
+ * crossline group) items, latest tiles of all volumes first, so uneven
+ * volumes balance; there is one queue per XCD (neighbouring items, which
+ * share aperture rows in that XCD's L2), and a workgroup whose XCD's queue
+ * is drained helps the others.  psm_batch_create packs nothing yet: it
  * plans every item and captures one HIP graph [queue reset, packing of each
  * distinct data cube, persistent kernel]; psm_batch_run replays it on the
  * context's stream (ms != NULL: waits and returns the elapsed time).
@@ -209,10 +211,15 @@ int psm_migrate_volume_dalpha(psm_ctx* ctx, const psm_survey* survey,
  * lives.  Item k's output equals psm_migrate_volume of that item at velocity
  * scale (alpha, alpha) by the same kernel form: the form follows the
  * context's psm_set_staging setting when the shape is set (create, set_shape,
- * autotune) -- staged for FP32 traces by default (workgroup width 8; the
- * waves_per_block argument is then only validated), gather otherwise.
- * psm_batch_autotune times candidate launch shapes (workgroup width x
- * resident workgroups per CU) on the batch itself and keeps the fastest.
+ * autotune) -- staged for FP32 traces by default (waves_per_block = output
+ * crosslines per workgroup, 4 or 8, other values 8; crossline offsets per
+ * staged group from psm_set_staging when staging is forced on, else 4),
+ * gather otherwise; the staged results do not depend on the shape.
+ * psm_batch_autotune times candidate launch shapes on the batch itself and
+ * keeps the fastest: workgroup width x resident workgroups per CU, and for
+ * the staged form also the offsets per staged group (2 or 4).
+ * psm_batch_get_shape reports the shape in use (staged: 1 for the staged
+ * form; steps: offsets per staged group, 0 for the gather form).
  * Storage: PSM_STORE_* for all items.
  */
 typedef struct psm_batch psm_batch;
@@ -228,6 +235,8 @@ int psm_batch_create(psm_ctx* ctx, const psm_batch_item* items, int n_items, int
 int psm_batch_set_shape(psm_batch* batch, int waves_per_block, int blocks_per_cu);
 int psm_batch_autotune(psm_batch* batch, int* waves_per_block, int* blocks_per_cu,
                        float* best_ms);
+int psm_batch_get_shape(psm_batch* batch, int* staged, int* waves_per_block,
+                        int* blocks_per_cu, int* steps);
 int psm_batch_run(psm_batch* batch, float* ms);
 void psm_batch_destroy(psm_batch* batch);
 

@@ -102,6 +102,8 @@ SIGNATURES = [
     ("psm_batch_set_shape", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("psm_batch_autotune", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_float)]),
+    ("psm_batch_get_shape", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("psm_batch_run", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     ("psm_batch_destroy", None, [C.c_void_p]),
     ("psm_table_velocity", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,

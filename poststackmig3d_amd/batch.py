@@ -48,6 +48,13 @@ class Batch:
         self.ctx.check(self.ctx.lib.psm_batch_set_shape(self.h, waves_per_block, blocks_per_cu),
                        "psm_batch_set_shape")
 
+    def shape(self):
+        """(staged, waves_per_block, blocks_per_cu, steps) of the launch in use."""
+        v = [C.c_int() for _ in range(4)]
+        self.ctx.check(self.ctx.lib.psm_batch_get_shape(self.h, *[C.byref(x) for x in v]),
+                       "psm_batch_get_shape")
+        return bool(v[0].value), v[1].value, v[2].value, v[3].value
+
     def autotune(self):
         w, b, ms = C.c_int(), C.c_int(), C.c_float()
         self.ctx.check(self.ctx.lib.psm_batch_autotune(self.h, C.byref(w), C.byref(b), C.byref(ms)),

@@ -817,9 +817,12 @@ int psm_last_timing(psm_ctx* c, psm_timing* t) {
 }  // extern "C"
 
 // ---------------------------------------------------------------- batches
-// Resident workgroups per CU of the staged batch form (its LDS: three 8-wave
-// workgroups of ~52 KB in 160 KB).
+// Resident workgroups per CU of the staged batch form by default: 8-wave
+// workgroups are held to three per CU by their registers (80 per lane at six
+// waves per SIMD), 4-wave workgroups to five by their LDS (about 30 KB each
+// with 4 offsets per group).
 constexpr int kBatchStagedPerCu = 3;
+constexpr int kBatchStaged4PerCu = 5;
 // Several volumes (a velocity scan over scaled Vrms models, sub-volumes of
 // any size) migrated by one persistent launch, captured once into a HIP graph
 // together with the trace packing and the queue reset, then replayed.
@@ -830,6 +833,7 @@ struct psm_batch {
   int n_items = 0;
   int wpb = 4, blocks_per_cu = 0;
   bool staged = false;  // kernel form of the captured launch
+  int stage_g = psm::kAutoStageSteps;  // staged form: crossline offsets per staged group
   int n_cu = 0;
   std::vector<psm::MigArgs> args;
   std::vector<psm::BatchSeg> segs;
@@ -864,7 +868,7 @@ int batch_capture(psm_batch* b) {
   batch_release_graph(b);
   hipStream_t st = c->own_stream;  // capture needs a stream of our own
   const int blocks = b->n_cu * (b->blocks_per_cu > 0 ? b->blocks_per_cu
-                                : b->staged     ? kBatchStagedPerCu
+                                : b->staged     ? (b->wpb == 4 ? kBatchStaged4PerCu : kBatchStagedPerCu)
                                                 : std::max(1, 32 / b->wpb));
   PSM_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   hipError_t e = psm::launch_fill_zero(b->counter.as<float>(), psm::kBatchXcds, st);
@@ -874,7 +878,7 @@ int batch_capture(psm_batch* b) {
                          g->n_t, st);
   }
   if (e == hipSuccess)
-    e = psm::launch_batch(b->storage, b->staged, b->d_args.as<psm::MigArgs>(),
+    e = psm::launch_batch(b->storage, b->staged, b->stage_g, b->d_args.as<psm::MigArgs>(),
                           b->d_segs.as<psm::BatchSeg>(), (int)b->segs.size(),
                           b->counter.as<unsigned>(), b->d_totals.as<unsigned>(), b->wpb, blocks,
                           st);
@@ -895,7 +899,9 @@ int batch_capture(psm_batch* b) {
 // the context's staging switch as psm_migrate_volume does: the staged form
 // (workgroups of kAutoStageWaves crosslines; wpb only validated) for FP32
 // traces by default or for any storage when forced on, else the gather form.
-int batch_set_shape(psm_batch* b, int wpb, int blocks_per_cu) {
+// stage_g: crossline offsets per staged group (2 or 4); <= 0: the context's
+// setting when staging is forced on, else kAutoStageSteps.
+int batch_set_shape(psm_batch* b, int wpb, int blocks_per_cu, int stage_g = 0) {
   psm_ctx* c = b->ctx;
   if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8)
     return fail(c, PSM_ERR_ARG, "batch waves_per_block must be 1, 2, 4 or 8");
@@ -905,7 +911,15 @@ int batch_set_shape(psm_batch* b, int wpb, int blocks_per_cu) {
   for (const psm::MigArgs& a : b->args)
     if ((size_t)(a.n_xd + 1) * a.n_t * (b->storage == PSM_STORE_F32 ? 8 : 4) > psm::kLdsMaxRowBytes)
       b->staged = false;
-  if (b->staged) wpb = psm::kAutoStageWaves;
+  if (b->staged) {
+    // FP32 traces: 4 or 8 output crosslines per workgroup, 2 or 4 offsets per
+    // group; FP16/BF16 (staging forced on): 8 and 4
+    const bool f32 = b->storage == PSM_STORE_F32;
+    wpb = (f32 && (wpb == 4 || wpb == 8)) ? wpb : psm::kAutoStageWaves;
+    if (stage_g != 2 && stage_g != 4)
+      stage_g = c->tune.staging > 0 ? c->tune.stage_steps : psm::kAutoStageSteps;
+    b->stage_g = f32 ? stage_g : psm::kAutoStageSteps;
+  }
   b->wpb = wpb;
   b->blocks_per_cu = blocks_per_cu;
   b->segs.clear();
@@ -1067,6 +1081,21 @@ int psm_batch_set_shape(psm_batch* b, int waves_per_block, int blocks_per_cu) {
   return batch_set_shape(b, waves_per_block, blocks_per_cu);
 }
 
+int psm_batch_get_shape(psm_batch* b, int* staged, int* waves_per_block, int* blocks_per_cu,
+                        int* steps) {
+  if (!b) return PSM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  const int wpb = b->wpb;
+  const int bpc = b->blocks_per_cu > 0 ? b->blocks_per_cu
+                  : b->staged         ? (wpb == 4 ? kBatchStaged4PerCu : kBatchStagedPerCu)
+                                      : std::max(1, 32 / wpb);
+  if (staged) *staged = b->staged ? 1 : 0;
+  if (waves_per_block) *waves_per_block = wpb;
+  if (blocks_per_cu) *blocks_per_cu = bpc;
+  if (steps) *steps = b->staged ? b->stage_g : 0;
+  return PSM_OK;
+}
+
 int psm_batch_run(psm_batch* b, float* ms) {
   if (!b) return PSM_ERR_ARG;
   std::lock_guard<std::mutex> lk(b->ctx->mu);
@@ -1078,19 +1107,21 @@ int psm_batch_autotune(psm_batch* b, int* waves_per_block, int* blocks_per_cu, f
   std::lock_guard<std::mutex> lk(b->ctx->mu);
   psm_ctx* c = b->ctx;
   PSM_HIP(c, hipSetDevice(c->device));
-  // candidate launch shapes: workgroup width x resident workgroups per CU
-  // (the staged form: its width, up to the workgroups its LDS lets a CU hold)
-  const int cand_gather[][2] = {{1, 32}, {2, 16}, {4, 8}, {8, 4}, {2, 8}, {4, 4}};
-  const int cand_staged[][2] = {{psm::kAutoStageWaves, kBatchStagedPerCu},
-                                {psm::kAutoStageWaves, kBatchStagedPerCu - 1}};
+  // candidate launch shapes: workgroup width (output crosslines) x resident
+  // workgroups per CU, and for the staged form of FP32 traces also the
+  // crossline offsets per staged group (the gather form: 0)
+  const int cand_gather[][3] = {{1, 32, 0}, {2, 16, 0}, {4, 8, 0}, {8, 4, 0}, {2, 8, 0}, {4, 4, 0}};
+  const int cand_staged[][3] = {{8, kBatchStagedPerCu, 4}, {8, kBatchStagedPerCu - 1, 4},
+                                {8, kBatchStagedPerCu, 2}, {4, kBatchStaged4PerCu, 4},
+                                {4, kBatchStaged4PerCu, 2}, {4, kBatchStaged4PerCu - 1, 4}};
   const bool staged = b->staged;  // the form batch_set_shape chose (it keeps it per shape)
-  const int n_cand = staged ? 2 : 6;
+  const int n_cand = staged ? (b->storage == PSM_STORE_F32 ? 6 : 2) : 6;
   float best = -1.0f;
-  int bw = 4, bb = 0;
+  int bw = 4, bb = 0, bg = 0;
   bool warm = false;
   for (int ci = 0; ci < n_cand; ++ci) {
     const int* cd = staged ? cand_staged[ci] : cand_gather[ci];
-    int rc = batch_set_shape(b, cd[0], cd[1]);
+    int rc = batch_set_shape(b, cd[0], cd[1], cd[2]);
     if (rc) return rc;
     float ms = 0.0f;
     if (!warm) {  // one untimed replay first: code object load, first-touch of the outputs
@@ -1104,9 +1135,10 @@ int psm_batch_autotune(psm_batch* b, int* waves_per_block, int* blocks_per_cu, f
       best = ms;
       bw = cd[0];
       bb = cd[1];
+      bg = cd[2];
     }
   }
-  const int rc = batch_set_shape(b, bw, bb);
+  const int rc = batch_set_shape(b, bw, bb, bg);
   if (rc) return rc;
   if (waves_per_block) *waves_per_block = bw;
   if (blocks_per_cu) *blocks_per_cu = bb;

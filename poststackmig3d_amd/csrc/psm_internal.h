@@ -131,10 +131,11 @@ hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream);
 // workgroups of wpb waves drain the kBatchXcds queues described by segs;
 // counters[x] (zero at launch) counts the positions taken from XCD x's queue,
 // totals[x] its length (both device memory).  staged: workgroups of the
-// staged form (kAutoStageWaves output crosslines, kAutoStageSteps offsets per
-// group; items of that width, wpb unused), else the gather form with wpb
+// staged form with wpb (4 or 8) output crosslines and stage_g (2 or 4)
+// offsets per group (FP16/BF16: 8 and 4 only), else the gather form with wpb
 // waves per workgroup.
-hipError_t launch_batch(int storage, bool staged, const MigArgs* items, const BatchSeg* segs,
+hipError_t launch_batch(int storage, bool staged, int stage_g, const MigArgs* items,
+                        const BatchSeg* segs,
                         int n_segs, unsigned* counters, const unsigned* totals, int wpb,
                         int blocks, hipStream_t stream);
 // out[0] = sum x[i] y[i] in float64; partial holds dot_scratch_doubles() - 1 doubles.

@@ -1860,15 +1860,32 @@ hipError_t launch_dot_f64(const float* x, const float* y, size_t n, double* part
 
 size_t dot_scratch_doubles() { return kDotBlocks + 1; }
 
+template <int ST, int W, int G>
+void launch_batch_lds(const MigArgs* items, const BatchSeg* segs, int n_segs, unsigned* counters,
+                      const unsigned* totals, int blocks, hipStream_t stream) {
+  hipLaunchKernelGGL((batch_lds_kernel<ST, W, G>), dim3((unsigned)blocks), dim3(kWave * W), 0,
+                     stream, items, segs, n_segs, counters, totals);
+}
+
 template <int ST>
-hipError_t launch_batch_st(bool staged, const MigArgs* items, const BatchSeg* segs, int n_segs,
-                           unsigned* counters, const unsigned* totals, int wpb, int blocks,
-                           hipStream_t stream) {
+hipError_t launch_batch_st(bool staged, int stage_g, const MigArgs* items, const BatchSeg* segs,
+                           int n_segs, unsigned* counters, const unsigned* totals, int wpb,
+                           int blocks, hipStream_t stream) {
   const dim3 grid((unsigned)blocks);
   if (staged) {
-    hipLaunchKernelGGL((batch_lds_kernel<ST, kAutoStageWaves, kAutoStageSteps>), grid,
-                       dim3(kWave * kAutoStageWaves), 0, stream, items, segs, n_segs, counters,
-                       totals);
+    // shapes the batch autotuner chooses from (FP32 traces); other storages
+    // stage with the automatic shape only
+    if constexpr (ST == PSM_STORE_F32) {
+      if (wpb == 4)
+        (stage_g == 2 ? launch_batch_lds<ST, 4, 2> : launch_batch_lds<ST, 4, 4>)(
+            items, segs, n_segs, counters, totals, blocks, stream);
+      else
+        (stage_g == 2 ? launch_batch_lds<ST, 8, 2> : launch_batch_lds<ST, 8, 4>)(
+            items, segs, n_segs, counters, totals, blocks, stream);
+    } else {
+      launch_batch_lds<ST, kAutoStageWaves, kAutoStageSteps>(items, segs, n_segs, counters,
+                                                             totals, blocks, stream);
+    }
     return hipGetLastError();
   }
   switch (wpb) {
@@ -1880,13 +1897,13 @@ hipError_t launch_batch_st(bool staged, const MigArgs* items, const BatchSeg* se
   return hipGetLastError();
 }
 
-hipError_t launch_batch(int storage, bool staged, const MigArgs* items, const BatchSeg* segs,
-                        int n_segs, unsigned* counters, const unsigned* totals, int wpb,
-                        int blocks, hipStream_t stream) {
+hipError_t launch_batch(int storage, bool staged, int stage_g, const MigArgs* items,
+                        const BatchSeg* segs, int n_segs, unsigned* counters,
+                        const unsigned* totals, int wpb, int blocks, hipStream_t stream) {
   switch (storage) {
-    case PSM_STORE_F16: return launch_batch_st<PSM_STORE_F16>(staged, items, segs, n_segs, counters, totals, wpb, blocks, stream);
-    case PSM_STORE_BF16: return launch_batch_st<PSM_STORE_BF16>(staged, items, segs, n_segs, counters, totals, wpb, blocks, stream);
-    default: return launch_batch_st<PSM_STORE_F32>(staged, items, segs, n_segs, counters, totals, wpb, blocks, stream);
+    case PSM_STORE_F16: return launch_batch_st<PSM_STORE_F16>(staged, stage_g, items, segs, n_segs, counters, totals, wpb, blocks, stream);
+    case PSM_STORE_BF16: return launch_batch_st<PSM_STORE_BF16>(staged, stage_g, items, segs, n_segs, counters, totals, wpb, blocks, stream);
+    default: return launch_batch_st<PSM_STORE_F32>(staged, stage_g, items, segs, n_segs, counters, totals, wpb, blocks, stream);
   }
 }
 hipError_t launch_table_velocity(const psm_velocity_point* pts, int n_pts, float vmin,

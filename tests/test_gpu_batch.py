@@ -81,7 +81,25 @@ def test_batch_matches_single_volumes(storage, staging):
             P.migrate_volume(items[0][1], items[0][2], items[0][0])
         w, bpc, ms = b.autotune()
         assert w in (1, 2, 4, 8) and ms > 0
-        assert not staged or w == 8
+        is_staged, w2, bpc2, steps = b.shape()
+        assert is_staged == staged and w2 == w and bpc2 == bpc
+        assert steps in ((2, 4) if staged else (0,))
+        assert not staged or w in ((4, 8) if storage == "f32" else (8,))
+        # every staged shape gives the same result: the forced shapes of the tuner
+        if staged and storage == "f32":
+            for ww, gg in ((4, 2), (8, 2), (4, 4)):
+                b.ctx.set_staging(True, gg)
+                b.set_shape(ww, 0)
+                assert b.shape()[1:] == (ww, 5 if ww == 4 else 3, gg)
+                for _, _, _, out, _ in items:
+                    out.fill_(float("nan"))
+                torch.cuda.synchronize()
+                b.run()
+                b.ctx.synchronize()
+                for k, (_, _, _, out, _) in enumerate(items):
+                    assert np.array_equal(out.cpu().numpy(), refs[k]), (ww, gg, k)
+            b.ctx.set_staging(staging)
+            b.set_shape(w, bpc)
         b.run()
         b.ctx.synchronize()
         for k, (_, _, _, out, _) in enumerate(items):
