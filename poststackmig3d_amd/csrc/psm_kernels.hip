@@ -695,8 +695,9 @@ struct StageTab {
 };
 
 // LDS of one staged workgroup.  Forward: slots of kWinPairs stored pairs.
-// Adjoint: slots of kWinPairs FP32 accumulators (data samples lo .. lo +
-// kWinPairs - 1) and a table of the samples each column flushes.
+// Adjoint: slots of kWinPairs accumulator pairs (A, B) -- the window's
+// samples lo .. lo + kWinPairs - 1 receive d[j] = A[j] - B[j] + B[j - 1] --
+// in G copies per column, and a table of the samples each column flushes.
 template <int ST, int W, int G, int MODE = kForward>
 struct LdsPlan {
   static constexpr bool kAdj = MODE == kAdjoint;
@@ -704,14 +705,21 @@ struct LdsPlan {
   // i, f, weight and mask, so a column's windows of the two rows are staged
   // as one window of summed pairs (x_p[i] + x_m[i], d_p[i] + d_m[i]) -- one
   // ring row, half the LDS reads and interpolations per step.  The adjoint
-  // likewise adds the same terms to both rows, so each wave accumulates one
-  // window per column and flushes it into both rows.  The per-trace scaled
-  // half-precision forward keeps one ring row per row.
+  // likewise adds the same terms to both rows, so one accumulator window per
+  // column is flushed into both rows.  The per-trace scaled half-precision
+  // forward keeps one ring row per row.
   static constexpr bool kSumRows = !kAdj && ST == PSM_STORE_F32;
   static constexpr int kRingRows = (kSumRows || kAdj) ? 1 : 2;
-  static constexpr int kB = kAdj ? 4 : Store<ST>::kBytes;  // bytes per pair / cell
+  static constexpr int kB = kAdj ? 8 : Store<ST>::kBytes;  // bytes per pair
   static constexpr int kSlot = kWinPairs * kB;           // bytes of one window
-  static constexpr int kCols = kMaxCols + 4 * G + W;     // table entries (reads past nO stay inside)
+  // adjoint: copy u of a column's window takes the adds of step u of a group
+  // (a column is touched by a different wave at each step, so waves that
+  // drift apart within a group never share a copy)
+  static constexpr int kCopies = kAdj ? G : 1;
+  // column offsets per side one row can stage (the adjoint's larger windows
+  // leave room for fewer table entries)
+  static constexpr int kMaxColsP = kAdj ? kMaxCols / 2 : kMaxCols;
+  static constexpr int kCols = kMaxColsP + 4 * G + W;    // table entries (reads past nO stay inside)
   static constexpr int kTables = (kAdj ? 5 : 3) * kCols * 4 + 5 * W * 4 + W * 4;
   // Double ring (2G + W - 1 slots per ring row and side): the next group's
   // windows are stored while the current group is read, one barrier per
@@ -719,32 +727,26 @@ struct LdsPlan {
   // that ends the current group's reads, two barriers per group; used when
   // the double ring would leave fewer than three workgroups per CU.
   static constexpr bool kSingle =
-      (2 * kRingRows * (2 * G + W - 1) + 1) * kSlot + kTables > 53 * 1024;
+      !kAdj && (2 * kRingRows * (2 * G + W - 1) + 1) * kSlot + kTables > 53 * 1024;
   static constexpr int kRing = kSingle ? G + W - 1 : 2 * G + W - 1;
-  static constexpr int kSide = kRing * kSlot;            // side 0 -> side 1
+  static constexpr int kSide = kRing * kSlot * kCopies;  // side 0 -> side 1
   static constexpr int kRow = 2 * kSide;                 // ring row 0 -> ring row 1
-  // one slot that absent staging jobs write; the adjoint's waves keep their
-  // windows in [0, W kSlot) instead of the ring
-  static constexpr int kSpare = kAdj ? W * kSlot : kRingRows * kRow;
-  static constexpr int kMeta = kSpare + kSlot;           // int meta[2][kCols]
+  // one slot that absent staging jobs write (forward)
+  static constexpr int kSpare = kRingRows * kRow;
+  static constexpr int kMeta = kSpare + (kAdj ? 0 : kSlot);  // int meta[2][kCols]
   static constexpr int kR2 = kMeta + 2 * kCols * 4;      // float r2tab[kCols]
   static constexpr int kPar = kR2 + kCols * 4;           // float par[5][W]
   static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
   static constexpr int kWid = kFlag + W * 4;             // adjoint: int wid[2][kCols]
-  static constexpr int kClaim = kWid + (kAdj ? 2 * kCols * 4 : 0);  // adjoint: int claim[W][kWinPairs]
-  // adjoint: float4 lp[W][kWave] = (t0^2, 4/v^2, R^2, weight factor) of every
-  // output crossline's lanes
-  static constexpr int kLP = (kClaim + (kAdj ? W * kWinPairs * 4 : 0) + 15) / 16 * 16;
-  // adjoint: float dummy[W][kWave], the cell a lane adds to in a pass it sits out
-  static constexpr int kDummy = kLP + (kAdj ? W * 64 * 16 : 0);
-  static constexpr int kBytes = kDummy + (kAdj ? W * 64 * 4 : 0);
+  static constexpr int kBytes = kWid + (kAdj ? 2 * kCols * 4 : 0);
   // waves per SIMD the LDS allows (three 8-wave workgroups per CU with the
   // double ring), else no register target
   static constexpr int kWavesPerEu = (W == 8 && !kSingle && 3 * (kBytes + 16) <= 160 * 1024) ? 6 : 1;
-  static_assert(kRow <= 65535, "ring row offset must fit a ds_read offset");
-  static_assert(kSlot <= 16 * kWave, "one load per window");
+  static_assert(kRingRows == 1 || kRow <= 65535, "ring row offset must fit a ds_read offset");
+  static_assert(kAdj || kSlot <= 16 * kWave, "one load per window");
   static_assert(!kAdj || !kSingle, "the adjoint uses the double ring");
-  __device__ static int slot(int side, int o) { return side * kSide + (o % kRing) * kSlot; }
+  static_assert(!kAdj || 3 * (kBytes + 16) <= 160 * 1024, "three adjoint workgroups per CU");
+  __device__ static int slot(int side, int o) { return side * kSide + (o % kRing) * kCopies * kSlot; }
 };
 
 
@@ -838,16 +840,12 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
       p.t0sq = gTlo2;
       p.a = gAlo;
     }
-    // adjoint: the parameters of every output crossline's lanes, published
-    // once through LDS and read back per step: (t0^2, 4/v^2, R^2, weight
-    // factor), the factor being (1/v^2) m with amplitude correction (the
-    // weight times the image sample is then rsq(tau^2) times it) and m
-    // without
-    using F4 = float __attribute__((ext_vector_type(4)));
-    F4* const lp4 = reinterpret_cast<F4*>(smem + PL::kLP);
+
     if constexpr (PL::kAdj) {
-      lp4[w * kWave + lane] = F4{p.t0sq, p.a, p.R2, AMP ? p.iv2 * p.mval : p.mval};
-      __syncthreads();
+      // the accumulator ring starts at zero; every flush leaves its slot zero
+      using Z4 = unsigned __attribute__((ext_vector_type(4)));
+      for (int off = (int)threadIdx.x * 16; off < PL::kRingRows * PL::kRow; off += W * kWave * 16)
+        *reinterpret_cast<Z4*>(smem + off) = Z4{0u, 0u, 0u, 0u};
     }
 
     int dmax = (int)(__builtin_sqrtf(R2wg) / a.dx) + 2;
@@ -895,7 +893,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
       const int r0 = hp ? rp : rm;  // ring row 0; ring row 1 = rm when both rows exist
       const bool two = hp && hm;
 
-      bool fallback = nO > kMaxCols;
+      bool fallback = nO > PL::kMaxColsP;
       if (!fallback) {
         __syncthreads();  // the previous row's windows and tables are no longer read
         // ---- column tables: meta[side][o] = LDS address of stored pair 0 of
@@ -938,12 +936,6 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           const float dyv = (float)dk * a.dy;
           r2tab[dk] = __builtin_fmaf(dyv, dyv, d2);
         }
-        if constexpr (PL::kAdj) {
-          // the accumulators of both ring rows start at zero
-          using Z4 = unsigned __attribute__((ext_vector_type(4)));
-          for (int off = (int)threadIdx.x * 16; off < W * PL::kSlot; off += W * kWave * 16)
-            *reinterpret_cast<Z4*>(smem + off) = Z4{0u, 0u, 0u, 0u};
-        }
         // (the vote runs with every lane active: lane l > 0 holds columns w +
         // W l, ...)
         const bool wave_ovf = __any(ovf);
@@ -953,133 +945,178 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
       }
       if constexpr (PL::kAdj) {
         if (!fallback) {
-          // ---- adjoint, column by column: each wave takes whole columns
-          // (side, o) and adds the terms of every output crossline of the
-          // workgroup that reads the column -- crossline v at offset dk = o -
-          // v on side 0, o - (W - 1 - v) on side 1, the same (i, f, w, mask)
-          // as the gather-form adjoint: d[i] += w (1 - f) m, d[i + 1] += w f m
-          // -- into a window of its own (samples lo .. lo + kWinPairs - 1 of
-          // the column's trace; lo as in the table, hi - lo <= kWinPairs - 2),
-          // then flushes the window into the data cube (global atomic adds of
-          // its nonzero samples) and zeroes it.  Rows +db and -db receive the
-          // same terms, so one window serves both and the flush adds it to
-          // each.  No window is shared between waves, so the sweep needs no
-          // barrier.
+          // ---- adjoint, in the forward's schedule: wave w at step dk adds the
+          // terms of its lanes -- the same (i, f, w, mask) as the gather-form
+          // adjoint: d[i] += w (1 - f) m, d[i + 1] += w f m -- to the windows
+          // of columns w + dk (side 0) and W-1-w + dk (side 1), one tau for
+          // the four pairs (+-db, +-dk).  Rows +db and -db receive the same
+          // terms, so one window per column serves both.  A window holds
+          // pairs (A, B): a term adds (w m, w f m) to pair i, and the flush
+          // forms d[j] = A[j] - B[j] + B[j - 1].  Step u of a group adds to
+          // copy u of the column's window: the W waves touch a column at W
+          // consecutive steps, so within a group no two waves touch the same
+          // copy, and the adds are plain read-add-write.  Column o is last
+          // touched at step dk = o (wave 0 on side 0, wave W-1 on side 1), so
+          // the G columns per side of a group are complete at its closing
+          // barrier; during the next group their copies are summed, flushed
+          // into the data cube (global atomic adds of the nonzero samples)
+          // and zeroed, and their slots are reused by the columns that enter
+          // the group after it (ring of 2G + W - 1 slots per side).
           const int* const wid = reinterpret_cast<const int*>(smem + PL::kWid);
-          using LdsInt = volatile __attribute__((address_space(3))) int;  // LDS, not flat
-          LdsInt* const claim = (LdsInt*)(smem + PL::kClaim) + w * kWinPairs;
-          const int wslot = w * PL::kSlot;  // this wave's window
+          using F2 = float __attribute__((ext_vector_type(2)));
+          using LdsF = __attribute__((address_space(3))) float;
+          // the lane's weight factor: (1/v^2) m with amplitude correction (the
+          // weight times the image sample is then rsq(tau^2) times it), m
+          // without
+          LaneParams pp = p;
+          pp.iv2 = AMP ? p.iv2 * p.mval : p.mval;
           const bool odd = (lane & 1) != 0;
-          const int dmy = PL::kDummy + (w * kWave + lane) * 4;  // this lane's dummy cell
-          auto add_cell = [&](int ar, float cc) {
-            float* const p0 = reinterpret_cast<float*>(smem + ar);
-            *p0 = *p0 + cc;
-          };
-          // one pass: the lanes in m add cc to their cell, the others to their
-          // dummy cell (no branch, so no exec-mask bookkeeping between passes)
-          auto pass = [&](bool m, int ar, float cc) {
-            float* const p0 = reinterpret_cast<float*>(smem + (m ? ar : dmy));
-            *p0 = *p0 + cc;
-          };
           // With one 4/v^2 over the workgroup's tile (laterally and
           // temporally constant velocity) q = sqrt(t0^2 + a r^2) / dt grows by
           // t0 / tau >= cos(theta) per lane inside the aperture, so at
-          // apertures below about 60 deg (cos > 1/2 with a margin) the fast
-          // path's condition holds for every lane with a term: no check.
+          // apertures below about 60 deg (cos > 1/2 with a margin) no two
+          // valid lanes of equal parity share an i: no check.
           const bool mono = gAlo == gAhi && a.alpha_mask * a.alpha_mask * a.tan2 <
                                                 2.9f * (a.alpha * a.alpha);
-          auto run_cols = [&](auto nr_c) {
-            constexpr int NR = decltype(nr_c)::value;
-            for (int job = w; job < 2 * nO; job += W) {
-              const int side = job & 1, o = job >> 1;
-              const int lo = uniform_i((PL::slot(side, o) - meta[side * PL::kCols + o]) / 4);
-              const int bse = wslot - lo * 4;
+          // pair i of both columns (one: side 0 only; the two never coincide)
+          auto rmw = [&](int ar, int al, bool one, float wm, float wf) {
+            F2* const cr = reinterpret_cast<F2*>(smem + ar);
+            F2* const cl = reinterpret_cast<F2*>(smem + al);
+            const F2 vr = *cr;
+            const F2 vl = one ? F2{0.0f, 0.0f} : *cl;
+            *cr = F2{vr.x + wm, vr.y + wf};
+            if (!one) *cl = F2{vl.x + wm, vl.y + wf};
+          };
+          auto lds_add = [&](int addr, float v) {
+            __hip_atomic_fetch_add((LdsF*)(smem + addr), v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+          };
+          // flush column o of `side` into each row: lane l forms the column's
+          // samples lo + l and lo + l + 64 from the G copies (each atomic
+          // instruction covers 256 contiguous bytes), then zeroes the copies
+          auto flush = [&](int side, int o) {
+            const int sl = PL::slot(side, o);
+            const int lo = uniform_i((sl - meta[side * PL::kCols + o]) / 8);
+            const int n = uniform_i(wid[side * PL::kCols + o]);
+            const int k = side == 0 ? kbase + o : kbase + (W - 1) - o;
+            float v0 = 0.0f, v1 = 0.0f;
 #pragma unroll
-              for (int v = 0; v < W; ++v) {
-                const int dk = side == 0 ? o - v : o - (W - 1 - v);
-                if (dk < side || dk > kmax_wg) continue;
-                const float r2 = r2tab[dk];
-                const F4 pv = lp4[v * kWave + lane];
-                const LaneParams pr{pv.x, pv.y, pv.z, pv.w, 0, 0, 0.0f};
-                const TauW tw = tau_weight(pr, r2, AMP);
-                const float q = tw.tau * a.inv_dt;
-                const int i = (int)q;
-                const bool on = r2 <= pv.z && i < nt1;
-                const float f = __builtin_amdgcn_fractf(q);
-                const float wm = AMP ? tw.w : pv.w;
-                const float c0 = wm - wm * f, c1 = wm * f;
-                int ar;
-                asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(ar) : "v"(i), "v"(bse));
-                // Fast path: i nondecreasing over the lanes with at most two
-                // lanes per value (the usual case: q advances by t0 / tau >=
-                // 1/2 per lane at apertures up to 60 deg).  Then no two even
-                // lanes share an i, nor two odd lanes, and the terms go in
-                // four read-add-write passes whose cells are distinct within
-                // each pass: cell i of the even lanes, of the odd lanes, then
-                // cell i + 1 of the even and of the odd lanes.  The passes are
-                // ordered (a wave's LDS accesses complete in issue order; the
-                // empty asm statements keep the compiler from moving them).
-                bool fast = mono;
-                if (!mono) {
-                  const int ip = lane_prev(i), ipp = lane_prev(ip);
-                  const bool dupp = lane > 0 && i == ip;
-                  fast = !__any((lane > 0 && i < ip) || (dupp && lane > 1 && ip == ipp));
-                }
-                if (fast) {
-                  asm volatile("" ::: "memory");
-                  pass(on && !odd, ar, c0);
-                  asm volatile("" ::: "memory");
-                  pass(on && odd, ar, c0);
-                  asm volatile("" ::: "memory");
-                  pass(on && !odd, ar + 4, c1);
-                  asm volatile("" ::: "memory");
-                  pass(on && odd, ar + 4, c1);
-                  asm volatile("" ::: "memory");
-                } else {
-                  // only lanes with a term take part (i of the others may lie
-                  // outside the window)
-                  const int ci = i & (kWinPairs - 1);
-                  bool pending = on;
-                  while (__any(pending)) {
-                    asm volatile("" ::: "memory");  // the previous round's adds come first
-                    if (pending) claim[ci] = lane;
-                    const bool win = pending && claim[ci] == lane;
-                    // samples i first, then i + 1 (the winner at i + 1 adds
-                    // there too): within each phase the winners' cells are
-                    // distinct
-                    if (win) add_cell(ar, c0);
-                    asm volatile("" ::: "memory");
-                    if (win) add_cell(ar + 4, c1);
-                    pending = pending && !win;
-                  }
-                }
-              }
-              asm volatile("" ::: "memory");  // the window is complete
-              // flush into each row: lane l holds the column's samples lo + l
-              // and lo + l + 64 (each atomic instruction covers 256
-              // contiguous bytes)
-              const int k = side == 0 ? kbase + o : kbase + (W - 1) - o;
-              const int n = wid[side * PL::kCols + o];
-              float* const cell = reinterpret_cast<float*>(smem + wslot);
-              const float v0 = cell[lane], v1 = cell[lane + kWave];
-              cell[lane] = 0.0f;
-              cell[lane + kWave] = 0.0f;
-              if (k >= xd_lo && k < xd_hi) {  // outside the data: nothing scattered there counts
+            for (int c = 0; c < G; ++c) {
+              const F2* const win = reinterpret_cast<const F2*>(smem + sl + c * PL::kSlot);
+              const F2 p0 = win[lane], p1 = win[lane + kWave];
+              const float bm0 = lane > 0 ? win[lane - 1].y : 0.0f;
+              const float bm1 = win[lane + kWave - 1].y;
+              v0 += (p0.x - p0.y) + bm0;
+              v1 += (p1.x - p1.y) + bm1;
+            }
+            asm volatile("" ::: "memory");  // every read before the zeroing
 #pragma unroll
-                for (int rr = 0; rr < NR; ++rr) {
-                  float* const dst = a.adj_out +
-                                     ((size_t)(rr == 0 ? r0 : rm) * a.n_xd + (k - xd_lo)) * a.n_t +
-                                     lo + lane;
-                  if (lane < n && v0 != 0.0f) unsafeAtomicAdd(dst, v0);
-                  if (lane + kWave < n && v1 != 0.0f) unsafeAtomicAdd(dst + kWave, v1);
-                }
+            for (int c = 0; c < G; ++c) {
+              F2* const win = reinterpret_cast<F2*>(smem + sl + c * PL::kSlot);
+              win[lane] = F2{0.0f, 0.0f};
+              win[lane + kWave] = F2{0.0f, 0.0f};
+            }
+            if (k >= xd_lo && k < xd_hi) {  // outside the data: nothing scattered there counts
+              float* const dst0 = a.adj_out + ((size_t)r0 * a.n_xd + (k - xd_lo)) * a.n_t + lo + lane;
+              if (lane < n && v0 != 0.0f) unsafeAtomicAdd(dst0, v0);
+              if (lane + kWave < n && v1 != 0.0f) unsafeAtomicAdd(dst0 + kWave, v1);
+              if (two) {
+                float* const dst1 = a.adj_out + ((size_t)rm * a.n_xd + (k - xd_lo)) * a.n_t + lo + lane;
+                if (lane < n && v0 != 0.0f) unsafeAtomicAdd(dst1, v0);
+                if (lane + kWave < n && v1 != 0.0f) unsafeAtomicAdd(dst1 + kWave, v1);
               }
             }
           };
-          if (two)
-            run_cols(std::integral_constant<int, 2>{});
+          // flush the columns [oa, ob) of both sides (jobs spread over the waves)
+          auto flush_cols = [&](int oa, int ob) {
+            for (int jb = w; jb < 2 * (ob - oa); jb += W) flush(jb & 1, oa + (jb >> 1));
+          };
+          using Tab = StageTab<G, 1>;
+          auto read_tab = [&](int dk0, Tab& t) {
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+              const int dk = dk0 + u;
+              t.r2[u] = r2tab[dk];
+              t.bR[u] = meta[w + dk] + u * PL::kSlot;
+              t.bL[u] = meta[PL::kCols + (W - 1 - w) + dk] + u * PL::kSlot;
+            }
+          };
+          // steps dk0 + u, u < n (FULL: all G); FIRST: dk = 0 has one column
+          auto compute = [&](auto full_c, auto first_c, auto mask_c, int dk0, int n, const Tab& t) {
+            constexpr bool MASK = decltype(mask_c)::value;
+            constexpr bool FULL = decltype(full_c)::value;
+            constexpr bool FIRST = decltype(first_c)::value;
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+              if (FULL || u < n) {
+                const bool one = FIRST && u == 0;
+                const float r2 = t.r2[u];
+                const TauW tw = tau_weight(pp, r2, AMP);
+                const float q = tw.tau * a.inv_dt;
+                const int iq = (int)q;
+                const int i = min(iq, nt1);  // (inside the window; terms at n_t - 1 are off)
+                bool on = valid && iq < nt1;
+                if constexpr (MASK) on = on && r2 <= p.R2;
+                const float f = __builtin_amdgcn_fractf(q);
+                const float wm = AMP ? tw.w : pp.iv2;
+                const float wf = wm * f;
+                int ar, al;
+                asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(ar) : "v"(i), "v"(t.bR[u]));
+                asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(al) : "v"(i), "v"(t.bL[u]));
+                bool fast = mono;
+                if (!mono) {
+                  // i nondecreasing over the valid lanes with at most two
+                  // lanes per value (lanes without a sample sort last)
+                  const int ik = valid ? iq : 0x40000000 + lane;
+                  const int ip = lane_prev(ik), ipp = lane_prev(ip);
+                  fast = !__any((lane > 0 && ik < ip) || (lane > 1 && ik == ip && ip == ipp));
+                }
+                if (fast) {
+                  // two passes, even lanes then odd lanes: within a pass no
+                  // two lanes share a pair
+                  if (on && !odd) rmw(ar, al, one, wm, wf);
+                  asm volatile("" ::: "memory");
+                  if (on && odd) rmw(ar, al, one, wm, wf);
+                } else if (on) {
+                  // lanes sharing a pair: LDS atomics (the copy is this wave's alone)
+                  lds_add(ar, wm);
+                  lds_add(ar + 4, wf);
+                  if (!one) {
+                    lds_add(al, wm);
+                    lds_add(al + 4, wf);
+                  }
+                }
+                asm volatile("" ::: "memory");
+              }
+            }
+          };
+          // one group: flush the previous group's columns, add this group's
+          // terms, read the next group's tables; the barrier completes the
+          // group's adds and the flushes' zeroing
+          auto run_group = [&](auto full_c, auto first_c, auto mask_c, int dk0, Tab& t) {
+            if (dk0 > 0) flush_cols(dk0 - G, min(dk0, nO));
+            compute(full_c, first_c, mask_c, dk0, min(G, kmax_wg + 1 - dk0), t);
+            read_tab(dk0 + G, t);
+            __syncthreads();
+          };
+          using T = std::integral_constant<bool, true>;
+          using F = std::integral_constant<bool, false>;
+          Tab t;
+          read_tab(0, t);
+          if (kmax_wg + 1 >= G)
+            run_group(T{}, T{}, T{}, 0, t);
           else
-            run_cols(std::integral_constant<int, 1>{});
+            run_group(F{}, T{}, T{}, 0, t);
+          int dk0 = G;
+          // groups inside every valid lane's aperture (dk <= kin) skip the mask
+          for (; dk0 + G <= min(kmax_wg, kin) + 1; dk0 += G) run_group(T{}, F{}, F{}, dk0, t);
+          for (; dk0 + G <= kmax_wg + 1; dk0 += G) run_group(T{}, F{}, T{}, dk0, t);
+          if (dk0 <= kmax_wg) {
+            run_group(F{}, F{}, T{}, dk0, t);
+            dk0 += G;
+          }
+          // the last group's columns and every column after it
+          flush_cols(dk0 - G, nO);
           continue;
         }
       } else if (!fallback) {
@@ -1761,10 +1798,12 @@ hipError_t launch_lds_wg(const MigArgs& a, hipStream_t stream) {
 }
 
 // staged adjoint: kAdjWaves output crosslines per workgroup (each column
-// window collects the terms of all of them before its one flush)
+// window collects the terms of all of them before its one flush), groups of
+// kAdjSteps crossline offsets
 constexpr int kAdjWaves = 8;
+constexpr int kAdjSteps = 2;
 hipError_t launch_adj_lds(const MigArgs& a, hipStream_t stream) {
-  constexpr int W = kAdjWaves, G = 1;
+  constexpr int W = kAdjWaves, G = kAdjSteps;
   const int n_tiles = (a.n_t + kWave - 1) / kWave;
   const int n_jgroups = (a.n_out_xl + W - 1) / W;
   const long long per_tile = (long long)a.n_out_il * n_jgroups;

@@ -13,12 +13,16 @@ Vectors live in HBM as float32 torch tensors (device memory plumbing); the
 inner products and residual norms are accumulated in float64 on the GPU
 (psm_dot_f64).  With `checkpoint=path` the full iteration state (m, r, p,
 gamma, iteration, residual history) is written every `checkpoint_every`
-iterations; `resume=path` continues from such a file.
+iterations; `resume=path` continues from such a file.  Checkpoints are
+written behind the solver: the state is copied into page-locked host buffers
+in stream order and a background thread writes the file, so only the last
+checkpoint of a call is waited for.
 """
 from __future__ import annotations
 
 import math
 import os
+import threading
 from dataclasses import dataclass, field
 
 from . import _lib
@@ -38,6 +42,83 @@ def _save(path, state):
     tmp = path + ".tmp"
     torch.save(state, tmp)
     os.replace(tmp, path)
+
+
+# page-locked host buffers kept across calls, keyed by (shape, dtype)
+_pinned: dict = {}
+_pinned_lock = threading.Lock()
+
+
+class _CheckpointWriter:
+    """Writes checkpoints without stalling the solver.  submit() copies the
+    device tensors of the state into page-locked host buffers on torch's
+    current stream -- in stream order, so the solver's later in-place updates
+    cannot overtake the copy -- and a background thread waits for the copy
+    and writes the file (to path.tmp, then renamed over path).  One write is
+    in flight at a time; wait() blocks until it is on disk and re-raises its
+    error, if any."""
+
+    def __init__(self, path):
+        self.path = path
+        self.thread = None
+        self.error = None
+        self.bufs = {}
+
+    def _buffer(self, name, t):
+        import torch
+        b = self.bufs.get(name)
+        if b is None or b.shape != t.shape or b.dtype != t.dtype:
+            key = (tuple(t.shape), t.dtype)
+            with _pinned_lock:
+                pool = _pinned.setdefault(key, [])
+                b = pool.pop() if pool else None
+            if b is None:
+                b = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+            self.bufs[name] = b
+        return b
+
+    def submit(self, state):
+        import torch
+        self.wait()
+        host = {}
+        for k, v in state.items():
+            if torch.is_tensor(v) and v.is_cuda:
+                b = self._buffer(k, v)
+                b.copy_(v, non_blocking=True)
+                host[k] = b
+            elif isinstance(v, list):
+                host[k] = list(v)
+            else:
+                host[k] = v
+        done = torch.cuda.Event()
+        done.record()
+
+        def write():
+            try:
+                done.synchronize()
+                _save(self.path, host)
+            except BaseException as e:  # re-raised by wait()
+                self.error = e
+
+        self.thread = threading.Thread(target=write, name="cgls-checkpoint", daemon=True)
+        self.thread.start()
+
+    def wait(self):
+        if self.thread is not None:
+            self.thread.join()
+            self.thread = None
+        if self.error is not None:
+            e, self.error = self.error, None
+            raise e
+
+    def close(self):
+        try:
+            self.wait()
+        finally:
+            with _pinned_lock:
+                for b in self.bufs.values():
+                    _pinned.setdefault((tuple(b.shape), b.dtype), []).append(b)
+            self.bufs = {}
 
 
 def cgls(data, vel, params: SurveyParams, iterations: int = 10, model0=None,
@@ -77,8 +158,21 @@ def cgls(data, vel, params: SurveyParams, iterations: int = 10, model0=None,
         rn = [math.sqrt(dot(r, r))]
         gn = [math.sqrt(gamma)]
 
+    writer = _CheckpointWriter(checkpoint) if checkpoint is not None else None
+    try:
+        k = _iterate(F, Ft, dot, m, r, p, gamma, k0, rn, gn, iterations, writer,
+                     checkpoint_every, stop_after)
+    finally:
+        if writer is not None:
+            writer.close()   # the last checkpoint is on disk before returning
+    return CGResult(model=m, residual_norms=rn, gradient_norms=gn, iterations=k)
+
+
+def _iterate(F, Ft, dot, m, r, p, gamma, k, rn, gn, iterations, writer, checkpoint_every,
+             stop_after):
+    """CGLS iterations k .. iterations - 1 (m, r, p updated in place, rn and gn
+    appended); returns the iteration count reached."""
     done_here = 0
-    k = k0
     while k < iterations:
         q = F(p)
         qq = dot(q, q)
@@ -96,9 +190,9 @@ def cgls(data, vel, params: SurveyParams, iterations: int = 10, model0=None,
         rn.append(math.sqrt(dot(r, r)))
         gn.append(math.sqrt(gamma))
         last = k == iterations or (stop_after is not None and done_here >= stop_after)
-        if checkpoint is not None and (k % checkpoint_every == 0 or last):
-            _save(checkpoint, {"m": m, "r": r, "p": p, "gamma": gamma, "iteration": k,
-                               "residual_norms": rn, "gradient_norms": gn})
+        if writer is not None and (k % checkpoint_every == 0 or last):
+            writer.submit({"m": m, "r": r, "p": p, "gamma": gamma, "iteration": k,
+                           "residual_norms": rn, "gradient_norms": gn})
         if stop_after is not None and done_here >= stop_after:
             break
-    return CGResult(model=m, residual_norms=rn, gradient_norms=gn, iterations=k)
+    return k
