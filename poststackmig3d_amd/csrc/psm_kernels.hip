@@ -851,6 +851,10 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
     int dmax = (int)(__builtin_sqrtf(R2wg) / a.dx) + 2;
     dmax = min(dmax, max(tk.b1 - 1 - c, c - tk.b0));
     dmax = uniform_i(min(dmax, a.d_hi - 1));
+    // true while no wave reads LDS of the previous row: a staged forward row
+    // ends with a barrier after its last reads (the parameter vote above and
+    // the adjoint's flushes do not)
+    bool quiet = false;
     for (int d = max(0, a.d_lo); d <= dmax; ++d) {
       const float dxv = (float)d * a.dx;
       const float d2 = uniform_f(dxv * dxv);
@@ -895,7 +899,8 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
 
       bool fallback = nO > PL::kMaxColsP;
       if (!fallback) {
-        __syncthreads();  // the previous row's windows and tables are no longer read
+        if (!quiet) __syncthreads();  // the previous row's windows and tables are no longer read
+        quiet = false;
         // ---- column tables: meta[side][o] = LDS address of stored pair 0 of
         // the column's window (slot - lo * kB); this wave fills o = w (mod W).
         // Column o of side s is read at the offsets dk in [max(s, o-W+1),
@@ -1443,6 +1448,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           run_row(std::integral_constant<int, 2>{});
         else
           run_row(std::integral_constant<int, 1>{});
+        quiet = true;
         continue;
       }
       // gather path for this row
