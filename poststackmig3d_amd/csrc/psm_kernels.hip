@@ -84,10 +84,18 @@ __device__ __forceinline__ float wave_min(float v) {
   return v;
 }
 
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
-  return v;
+// Largest v over lanes 0 .. N-1 (N a power of two, at most 16), as a
+// scalar: DPP row_shr:1, 2, 4, 8 inside the first row of 16 lanes (a lane
+// whose source lies before its row keeps its own value), then lane N-1.
+// Called with every lane active.
+template <int N>
+__device__ __forceinline__ int lanes_max_i(int v) {
+  static_assert(N >= 1 && N <= 16 && (N & (N - 1)) == 0, "N lanes of one DPP row");
+  if constexpr (N >= 2) v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xf, 0xf, false));
+  if constexpr (N >= 4) v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xf, 0xf, false));
+  if constexpr (N >= 8) v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false));
+  if constexpr (N >= 16) v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false));
+  return __builtin_amdgcn_readlane(v, N - 1);
 }
 
 __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
@@ -99,10 +107,6 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
 // Lane l receives v of lane l - 1 (lane 0 receives 0): DPP wave_shr:1.
 __device__ __forceinline__ int lane_prev(int v) {
   return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
-}
-// Lane l receives v of lane l + 1 (lane 63 receives 0): DPP wave_shl:1.
-__device__ __forceinline__ int lane_next(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true);
 }
 
 __device__ __forceinline__ float uniform_f(float v) {
@@ -243,7 +247,8 @@ __device__ __forceinline__ void sweep_row(const MigArgs& a, int kj, const LanePa
   const int dk_lim = max(kj - xd_lo, xd_hi - 1 - kj);
 
   // last dk with r^2 <= R2max (r^2 is nondecreasing in dk)
-  int kmax = (int)(__builtin_sqrtf(R2max - d2) * a.inv_dy) + 2;
+  // (v_sqrt_f32 is within one ulp; the loop finds the last offset)
+  int kmax = (int)(__builtin_amdgcn_sqrtf(R2max - d2) * a.inv_dy) + 2;
   kmax = min(kmax, dk_lim);
   while (kmax >= 0) {
     const float dyv = (float)kmax * a.dy;
@@ -855,6 +860,19 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
     // ends with a barrier after its last reads (the parameter vote above and
     // the adjoint's flushes do not)
     bool quiet = false;
+    // Last crossline offsets carried from row to row: the test r^2 =
+    // fma(dk dy, dk dy, d^2) <= R^2 only gets harder as d grows, so each
+    // row resumes from the previous row's answer and steps down.  km_run:
+    // lane l < W, wave l's sweep (clipped to offsets that can hold data);
+    // kin_run: every valid lane of this wave inside the aperture.
+    int km_run = -1;
+    if (pl && pR2 >= 0.0f) {
+      const int kjl = kbase + lane;
+      // (v_sqrt_f32 is within one ulp: the start is at or above the answer)
+      km_run = min((int)(__builtin_amdgcn_sqrtf(pR2) * a.inv_dy) + 2,
+                   max(kjl - xd_lo, xd_hi - 1 - kjl));
+    }
+    int kin_run = R2in >= 0.0f ? (int)(__builtin_amdgcn_sqrtf(R2in) * a.inv_dy) + 2 : -1;
     for (int d = max(0, a.d_lo); d <= dmax; ++d) {
       const float dxv = (float)d * a.dx;
       const float d2 = uniform_f(dxv * dxv);
@@ -865,35 +883,27 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
       if (!hp && !hm) continue;
       // last crossline offset of each wave's sweep (lane l < W: wave l), as
       // in sweep_row: r^2 <= R2max, clipped to offsets that can hold data
-      int km = -1;
-      if (pl && pR2 >= d2) {
-        const int kjl = kbase + lane;
-        int k = (int)(__builtin_sqrtf(pR2 - d2) * a.inv_dy) + 2;
-        k = min(k, max(kjl - xd_lo, xd_hi - 1 - kjl));
-        while (k >= 0) {
-          const float dyv = (float)k * a.dy;
+      if (pl) {
+        if (pR2 < d2) km_run = -1;
+        while (km_run >= 0) {
+          const float dyv = (float)km_run * a.dy;
           if (__builtin_fmaf(dyv, dyv, d2) <= pR2) break;
-          --k;
+          --km_run;
         }
-        km = k;
       }
-      const int kmax_wg = uniform_i(wave_max_i(km));
+      const int kmax_wg = lanes_max_i<W>(km_run);
       if (kmax_wg < 0) continue;
       const int nO = kmax_wg + W;  // column offsets per side
       // last crossline offset inside every valid lane's aperture (r^2 as in
       // the column table), at most kmax_wg; -1 if none
-      int kin = -1;
-      if (R2in >= d2) {
-        const float kf = __builtin_sqrtf(R2in - d2) * a.inv_dy;
-        int k = kf >= (float)kmax_wg ? kmax_wg : (int)kf + 1;
-        while (k >= 0) {
-          const float dyv = (float)k * a.dy;
-          if (__builtin_fmaf(dyv, dyv, d2) <= R2in) break;
-          --k;
-        }
-        kin = k;
+      // (kmax_wg is nonincreasing in d as well, so the clip carries over)
+      kin_run = R2in < d2 ? -1 : min(kin_run, kmax_wg);
+      while (kin_run >= 0) {
+        const float dyv = (float)kin_run * a.dy;
+        if (__builtin_fmaf(dyv, dyv, d2) <= R2in) break;
+        --kin_run;
       }
-      kin = uniform_i(kin);
+      const int kin = uniform_i(kin_run);
       const int r0 = hp ? rp : rm;  // ring row 0; ring row 1 = rm when both rows exist
       const bool two = hp && hm;
 
@@ -918,11 +928,12 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
               int lo = 0;
               if (dlo <= dhi) {
                 const float ylo = (float)dlo * a.dy, yhi = (float)dhi * a.dy;
-                const float lo_t = __builtin_sqrtf(
+                const float lo_t = __builtin_amdgcn_sqrtf(
                     __builtin_fmaf(gAlo, __builtin_fmaf(ylo, ylo, d2), gTlo2));
-                const float hi_t = __builtin_sqrtf(
+                const float hi_t = __builtin_amdgcn_sqrtf(
                     __builtin_fmaf(gAhi, __builtin_fmaf(yhi, yhi, d2), gThi2));
-                // one index of slack each way covers the rounding of q
+                // one index of slack each way covers the rounding of q and
+                // the one-ulp error of v_sqrt_f32 (a relative 1.2e-7 of q)
                 lo = min(max((int)(lo_t * a.inv_dt) - 1, 0), nt1);
                 const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
                 // (the adjoint also touches sample hi + 1)
