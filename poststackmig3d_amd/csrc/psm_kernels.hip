@@ -845,6 +845,12 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
       p.t0sq = gTlo2;
       p.a = gAlo;
     }
+    // Forward with amplitude correction: every term's weight is
+    // rsq(tau^2) * (1/v^2) with the lane's one 1/v^2, so the sweep (staged
+    // groups and gather rows alike) accumulates sum * rsq(tau^2) and the
+    // lane's 1/v^2 multiplies the total once, at the end.
+    const float lane_iv2 = p.iv2;
+    if constexpr (MODE == kForward && AMP) p.iv2 = 1.0f;
 
     if constexpr (PL::kAdj) {
       // the accumulator ring starts at zero; every flush leaves its slot zero
@@ -1470,6 +1476,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                                        hm ? base + (size_t)rm * row_bytes : zero_row, hm,
                                        hm ? rm : a.n_rows, d2);
     }
+    if constexpr (MODE == kForward && AMP) acc *= lane_iv2;
   }
   if constexpr (MODE == kForward) {
     if (in_out && active) {
