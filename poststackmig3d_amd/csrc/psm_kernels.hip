@@ -918,15 +918,18 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         if (!quiet) __syncthreads();  // the previous row's windows and tables are no longer read
         quiet = false;
         // ---- column tables: meta[side][o] = LDS address of stored pair 0 of
-        // the column's window (slot - lo * kB); this wave fills o = w (mod W).
+        // the column's window (slot - lo * kB); wave w fills columns
+        // 64 w .. 64 w + 63 (and 64 (w + W) ... when a row has more than
+        // 64 W), so only the first nO / 64 waves spend instructions on it and
+        // the table writes are conflict-free.
         // Column o of side s is read at the offsets dk in [max(s, o-W+1),
         // min(o, kmax)] (one per wave), so q of every read lies between
         // sqrt(t0lo^2 + alo r^2(dk_lo)) / dt and sqrt(t0hi^2 + ahi r^2(dk_hi)) / dt
         // with the workgroup's bounds of t0^2 and a = 4/v^2.  r2tab[dk] =
         // r^2 of the row pair at crossline offset dk.
         bool ovf = false;
-        for (int ob = w; ob < nO; ob += W * kWave) {
-          const int o = ob + W * lane;
+        for (int ob = w * kWave; ob < nO; ob += W * kWave) {
+          const int o = ob + lane;
           if (o < nO) {
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
