@@ -696,7 +696,7 @@ template <int G, int J>
 struct StageTab {
   float r2[G];
   int bR[G], bL[G];
-  int mj[J];
+  int mj[J], mw[J];
 };
 
 // LDS of one staged workgroup.  Forward: slots of kWinPairs stored pairs.
@@ -725,7 +725,7 @@ struct LdsPlan {
   // leave room for fewer table entries)
   static constexpr int kMaxColsP = kAdj ? kMaxCols / 2 : kMaxCols;
   static constexpr int kCols = kMaxColsP + 4 * G + W;    // table entries (reads past nO stay inside)
-  static constexpr int kTables = (kAdj ? 5 : 3) * kCols * 4 + 5 * W * 4 + W * 4;
+  static constexpr int kTables = 5 * kCols * 4 + 5 * W * 4 + W * 4;
   // Double ring (2G + W - 1 slots per ring row and side): the next group's
   // windows are stored while the current group is read, one barrier per
   // group.  Single ring (G + W - 1 slots): they are stored after a barrier
@@ -742,8 +742,10 @@ struct LdsPlan {
   static constexpr int kR2 = kMeta + 2 * kCols * 4;      // float r2tab[kCols]
   static constexpr int kPar = kR2 + kCols * 4;           // float par[5][W]
   static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
-  static constexpr int kWid = kFlag + W * 4;             // adjoint: int wid[2][kCols]
-  static constexpr int kBytes = kWid + (kAdj ? 2 * kCols * 4 : 0);
+  // int wid[2][kCols]: forward, bytes of the column's window (pairs lo ..
+  // hi); adjoint, samples the column flushes
+  static constexpr int kWid = kFlag + W * 4;
+  static constexpr int kBytes = kWid + 2 * kCols * 4;
   // waves per SIMD the LDS allows (three 8-wave workgroups per CU with the
   // double ring), else no register target
   static constexpr int kWavesPerEu = (W == 8 && !kSingle && 3 * (kBytes + 16) <= 160 * 1024) ? 6 : 1;
@@ -928,9 +930,18 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // with the workgroup's bounds of t0^2 and a = 4/v^2.  r2tab[dk] =
         // r^2 of the row pair at crossline offset dk.
         bool ovf = false;
-        for (int ob = w * kWave; ob < nO; ob += W * kWave) {
+        // The staging jobs of a row read entries up to nO + 3G - 1 (loads
+        // issued for columns past the row store into the spare slot); their
+        // widths are zeroed, so those loads read nothing.
+        constexpr int kPast = 4 * G + W;
+        for (int ob = w * kWave; ob < nO + kPast; ob += W * kWave) {
           const int o = ob + lane;
-          if (o < nO) {
+          if (o >= nO) {
+            if (o < nO + kPast) {
+              reinterpret_cast<int*>(smem + PL::kWid)[o] = 0;
+              reinterpret_cast<int*>(smem + PL::kWid)[PL::kCols + o] = 0;
+            }
+          } else {
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
               const int dlo = max(side, o - (W - 1)), dhi = min(o, kmax_wg);
@@ -947,10 +958,9 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                 const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
                 // (the adjoint also touches sample hi + 1)
                 ovf = ovf || (hi - lo >= (PL::kAdj ? kWinPairs - 1 : kWinPairs));
-                if constexpr (PL::kAdj)
-                  reinterpret_cast<int*>(smem + PL::kWid)[side * PL::kCols + o] =
-                      min(hi + 1, nt1) - lo + 1;  // samples the column flushes
-              } else if constexpr (PL::kAdj) {
+                reinterpret_cast<int*>(smem + PL::kWid)[side * PL::kCols + o] =
+                    PL::kAdj ? min(hi + 1, nt1) - lo + 1 : (hi - lo + 1) * kB;
+              } else {
                 reinterpret_cast<int*>(smem + PL::kWid)[side * PL::kCols + o] = 0;
               }
               meta[side * PL::kCols + o] = PL::slot(side, o) - lo * kB;
@@ -1182,6 +1192,9 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         };
         auto job_dst = [&](const StageJob& jb) { return PL::slot(jb.side, jb.o) + jb.rr * PL::kRow; };
         auto tab = [&](int side, int o) { return meta[side * PL::kCols + o]; };
+        auto wtab = [&](int side, int o) {
+          return reinterpret_cast<const int*>(smem + PL::kWid)[side * PL::kCols + o];
+        };
         {  // the first G + W - 1 columns, loaded straight into LDS (no
            // registers held); complete before the barrier
           constexpr int kFill = (2 * PL::kRingRows * (G + W - 1) + W - 1) / W;
@@ -1264,7 +1277,10 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           // table entries of the next loads (read whether or not the job
           // exists: the index stays inside the table)
 #pragma unroll
-          for (int q = 0; q < kJobs; ++q) t.mj[q] = tab(js[q], jo[q]);
+          for (int q = 0; q < kJobs; ++q) {
+            t.mj[q] = tab(js[q], jo[q]);
+            t.mw[q] = wtab(js[q], jo[q]);
+          }
         };
         // Steps dk0 + u, u < n (FULL: all G).  Every wave sweeps the
         // workgroup's offsets: each column's window covers all waves' reads up
@@ -1364,7 +1380,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // (summed rows: the second row's loads follow the first's, v[kJobs + q])
         using Jobs = V4[PL::kSumRows ? 2 * kJobs : kJobs];
         using Dsts = int[kJobs];
-        auto issue = [&](auto nr_c, const int* m, Jobs& v, Dsts& dst) {
+        auto issue = [&](auto nr_c, const int* m, const int* mw, Jobs& v, Dsts& dst) {
           constexpr int NR = decltype(nr_c)::value;
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
@@ -1377,7 +1393,9 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
             const bool data = (unsigned)jkd[q] < (unsigned)a.n_xd;
             dst[q] = jo[q] < jlim[q] ? jsl[q] : PL::kSpare;
             const int voff = jsl[q] + jlbs[q] - m[q];  // lo * kB + lane * 16
-            const int vo = voff > nt1 * kB ? 0x7ff00000 : voff;
+            // (lanes past the window's pairs lo .. hi -- hi <= n_t - 1 --
+            // load nothing)
+            const int vo = lb < mw[q] ? voff : 0x7ff00000;
             const int so = data ? jso[q] : a.n_xd * tb;
             v[q] = __builtin_amdgcn_raw_buffer_load_b128(jrs[q], vo, so, 0);
             if constexpr (PL::kSumRows && NR == 2)
@@ -1398,7 +1416,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           // stores and barrier -- is not held up behind other waves'
           // sums; 0.75 % on the production survey)
           __builtin_amdgcn_s_setprio(1);
-          issue(nr_c, t.mj, vi, di);
+          issue(nr_c, t.mj, t.mw, vi, di);
           __builtin_amdgcn_s_setprio(0);
           // (summed rows: one ring row holds both rows of the pair)
           constexpr int NR = decltype(nr_c)::value;
@@ -1433,10 +1451,13 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           Jobs va, vb;
           Dsts da, db;
           {  // the loads stored at the end of group 0
-            int m0[kJobs];
+            int m0[kJobs], w0[kJobs];
 #pragma unroll
-            for (int q = 0; q < kJobs; ++q) m0[q] = tab(js[q], jo[q]);
-            issue(nr_c, m0, va, da);
+            for (int q = 0; q < kJobs; ++q) {
+              m0[q] = tab(js[q], jo[q]);
+              w0[q] = wtab(js[q], jo[q]);
+            }
+            issue(nr_c, m0, w0, va, da);
           }
           Tab t;
           read_tab(0, t);
