@@ -1182,12 +1182,14 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           return StageJob{oa + (j >> jsh), (j >> (jsh - 1)) & 1, j & (jsh - 1), j < (nc << jsh)};
         };
         // buffer offsets of a job whose table entry is m (meta of its column)
-        auto job_src = [&](const StageJob& jb, int m, int& voff, int& soff) {
+        // (mw = the column's window bytes: lanes past pairs lo .. hi load
+        // nothing)
+        auto job_src = [&](const StageJob& jb, int m, int mw, int& voff, int& soff) {
           const int k = jb.side == 0 ? kbase + jb.o : kbase + (W - 1) - jb.o;
           const bool data = k >= xd_lo && k < xd_hi;
           const int lo = uniform_i((PL::slot(jb.side, jb.o) - m) / kB);
           soff = data ? (k - xd_lo) * tb : 0;
-          const bool zero = !data || lo * kB + lb > nt1 * kB;
+          const bool zero = !data || lb >= mw;
           voff = zero ? 0x7ffffff0 : lo * kB + lb;
         };
         auto job_dst = [&](const StageJob& jb) { return PL::slot(jb.side, jb.o) + jb.rr * PL::kRow; };
@@ -1199,18 +1201,19 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
            // registers held); complete before the barrier
           constexpr int kFill = (2 * PL::kRingRows * (G + W - 1) + W - 1) / W;
           const int nc = min(G + W - 1, nO);
-          int m[kFill];
+          int m[kFill], mw[kFill];
 #pragma unroll
           for (int t = 0; t < kFill; ++t) {
             const StageJob jb = job_at(0, nc, w + t * W);
             m[t] = jb.ok ? tab(jb.side, jb.o) : 0;
+            mw[t] = jb.ok ? wtab(jb.side, jb.o) : 0;
           }
 #pragma unroll
           for (int t = 0; t < kFill; ++t) {
             const StageJob jb = job_at(0, nc, w + t * W);
             if (jb.ok && lb < PL::kSlot) {
               int voff, soff;
-              job_src(jb, m[t], voff, soff);
+              job_src(jb, m[t], mw[t], voff, soff);
               if (PL::kSumRows && two) {
                 // both rows through registers, their sum into the slot
                 const F4 x0 = __builtin_bit_cast(
