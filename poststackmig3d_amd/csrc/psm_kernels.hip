@@ -715,6 +715,10 @@ struct LdsPlan {
   // forward keeps one ring row per row.
   static constexpr bool kSumRows = !kAdj && ST == PSM_STORE_F32;
   static constexpr int kRingRows = (kSumRows || kAdj) ? 1 : 2;
+  // FP32 forward: a window may run past the trace end -- its pairs past
+  // n_t - 1 are zeros in the slot -- so the steps read pair i without
+  // clamping it (two 8-byte pairs per 16-byte staging lane)
+  static constexpr bool kOpenEnd = kSumRows;
   static constexpr int kB = kAdj ? 8 : Store<ST>::kBytes;  // bytes per pair
   static constexpr int kSlot = kWinPairs * kB;           // bytes of one window
   // adjoint: copy u of a column's window takes the adds of step u of a group
@@ -954,12 +958,31 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
                     __builtin_fmaf(gAhi, __builtin_fmaf(yhi, yhi, d2), gThi2));
                 // one index of slack each way covers the rounding of q and
                 // the one-ulp error of v_sqrt_f32 (a relative 1.2e-7 of q)
-                lo = min(max((int)(lo_t * a.inv_dt) - 1, 0), nt1);
-                const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
-                // (the adjoint also touches sample hi + 1)
-                ovf = ovf || (hi - lo >= (PL::kAdj ? kWinPairs - 1 : kWinPairs));
-                reinterpret_cast<int*>(smem + PL::kWid)[side * PL::kCols + o] =
-                    PL::kAdj ? min(hi + 1, nt1) - lo + 1 : (hi - lo + 1) * kB;
+                int* const wid = reinterpret_cast<int*>(smem + PL::kWid) + side * PL::kCols + o;
+                if constexpr (PL::kOpenEnd) {
+                  // The window is not cut at the trace end; only pairs up to
+                  // n_t - 1 are loaded and the lanes past them load zeros.
+                  // The lane holding pair n_t - 1 must not also hold pair n_t
+                  // (the next trace's first), so where the window reaches
+                  // past the end lo is lowered by one to make n_t - 1 - lo
+                  // odd.
+                  lo = max((int)(lo_t * a.inv_dt) - 1, 0);
+                  const int hi = max((int)(hi_t * a.inv_dt) + 1, 0);
+                  if (hi > nt1 && ((nt1 - lo) & 1) == 0) {
+                    if (lo > 0)
+                      --lo;
+                    else
+                      ovf = true;
+                  }
+                  ovf = ovf || (hi - lo >= kWinPairs);
+                  *wid = lo <= nt1 ? (min(hi, nt1) - lo + 1) * kB : 0;
+                } else {
+                  lo = min(max((int)(lo_t * a.inv_dt) - 1, 0), nt1);
+                  const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
+                  // (the adjoint also touches sample hi + 1)
+                  ovf = ovf || (hi - lo >= (PL::kAdj ? kWinPairs - 1 : kWinPairs));
+                  *wid = PL::kAdj ? min(hi + 1, nt1) - lo + 1 : (hi - lo + 1) * kB;
+                }
               } else {
                 reinterpret_cast<int*>(smem + PL::kWid)[side * PL::kCols + o] = 0;
               }
@@ -971,8 +994,8 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           const float dyv = (float)dk * a.dy;
           r2tab[dk] = __builtin_fmaf(dyv, dyv, d2);
         }
-        // (the vote runs with every lane active: lane l > 0 holds columns w +
-        // W l, ...)
+        // (the vote runs with every lane active: lane l holds columns 64 w +
+        // l, ...)
         const bool wave_ovf = __any(ovf);
         if (lane == 0) flag[w] = wave_ovf ? 1 : 0;
         __syncthreads();  // tables and flags are visible
@@ -1313,7 +1336,8 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
               const float q = tw.tau * a.inv_dt;
               // both column addresses as (i << log2 kB) + base, one
               // instruction each (the compiler shares the shift otherwise)
-              const int i = min((int)q, nt1);
+              // (a lane past the trace end reads a zero pair of the window)
+              const int i = PL::kOpenEnd ? (int)q : min((int)q, nt1);
               int ar, al;
               asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(ar) : "v"(i), "i"(S::kShift), "v"(t.bR[u]));
               asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(al) : "v"(i), "i"(S::kShift), "v"(t.bL[u]));
