@@ -7,8 +7,9 @@ reference kernel to the parity bar (1e-5).  The cases reach every path of the
 staged kernel: sticky trailing crosslines (the workgroup runs the gather
 path), coarse crossline bins with a slow velocity (time windows wider than a
 staged slot: per-row fallback), fine crossline bins (more column offsets than
-the column table holds: per-row fallback), traces shorter than a slot, and
-both storage forms.
+the column table holds: per-row fallback), traces shorter than a slot, large
+samples at both ends of every trace (windows that run past the trace end),
+and both storage forms.
 """
 import numpy as np
 import pytest
@@ -69,6 +70,20 @@ def _case(name):
         vel = synth.production_velocity(*n)
         geo = dict(inline_step=25.0, crossline_step=25.0, inline_padding=2, crossline_padding=3,
                    angle_aperture=60.0)
+    elif name in ("end_spikes_even", "end_spikes_odd"):
+        # every trace starts with a large sample and ends with two: a term
+        # that read past the last pair (the next trace's first pair, or a
+        # pair the bound i < n_t - 1 excludes) would stand out far above the
+        # bar; n_t - 1 even and odd, since the staged windows that run past
+        # the trace end are aligned on it
+        n = (19, 21, 301 if name == "end_spikes_even" else 300)
+        data, _ = synth.production(*n, n_diffractors=3, seed=10)
+        data[:, :, 0] = 1000.0
+        data[:, :, -1] = 100.0
+        data[:, :, -2] = -50.0
+        vel = synth.production_velocity(*n)
+        geo = dict(inline_step=25.0, crossline_step=25.0, inline_padding=2, crossline_padding=3,
+                   angle_aperture=45.0)
     else:
         raise KeyError(name)
     prm = P.SurveyParams.for_cube(n, **geo)
@@ -78,7 +93,7 @@ def _case(name):
 
 
 CASE_NAMES = ["padded_lateral", "point_unpadded", "coarse_slow", "mid_bins", "fine_bins",
-              "short_traces"]
+              "short_traces", "end_spikes_even", "end_spikes_odd"]
 
 
 @pytest.fixture(scope="module")
