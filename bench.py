@@ -102,8 +102,9 @@ def parse():
     ap.add_argument("--storage", choices=["f32", "f16", "bf16"], default="f32",
                     help="trace storage of the main line (compute stays FP32)")
     ap.add_argument("--kernel", choices=["auto", "staged", "gather"], default="auto",
-                    help="forward kernel form: chosen by trace storage (auto), aperture "
-                         "windows staged in LDS (8 crosslines per workgroup, --stage-steps offsets per "
+                    help="forward kernel form: the library's default (auto: staged, 8 "
+                         "crosslines per workgroup, 4 offsets per group), aperture windows "
+                         "staged in LDS (8 crosslines per workgroup, --stage-steps offsets per "
                          "group, one launch), or gathered through the vector memory path")
     ap.add_argument("--stage-steps", type=int, choices=[2, 4], default=4,
                     help="crossline offsets per staged group with --kernel staged")
@@ -744,10 +745,10 @@ def main():
     avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
     gpu_img = out.cpu().numpy()
 
-    staged = args.kernel == "staged" or (args.kernel == "auto" and args.storage == "f32")
+    staged = args.kernel != "gather"
     form = "staged" if staged else "gather"
-    kernel_name = "%s (forward, FP%s traces)" % ("migrate_lds_kernel" if staged else "migrate_kernel",
-                                                 "32" if args.storage == "f32" else "16")
+    kernel_name = "%s (forward, %s traces)" % ("migrate_lds_kernel" if staged else "migrate_kernel",
+                                               args.storage.upper())
     pair_bytes = 8.0 if args.storage == "f32" else 4.0
 
     variants = {}

@@ -211,10 +211,11 @@ int psm_migrate_volume_dalpha(psm_ctx* ctx, const psm_survey* survey,
  * lives.  Item k's output equals psm_migrate_volume of that item at velocity
  * scale (alpha, alpha) by the same kernel form: the form follows the
  * context's psm_set_staging setting when the shape is set (create, set_shape,
- * autotune) -- staged for FP32 traces by default (waves_per_block = output
- * crosslines per workgroup, 4 or 8, other values 8; crossline offsets per
- * staged group from psm_set_staging when staging is forced on, else 4),
- * gather otherwise; the staged results do not depend on the shape.
+ * autotune) -- staged by default (waves_per_block = output crosslines per
+ * workgroup, 4 or 8, other values 8; crossline offsets per staged group from
+ * psm_set_staging when staging is forced on, else 4), gather when staging is
+ * switched off or a pair-layout row exceeds 1 GiB; the staged results do not
+ * depend on the shape.
  * psm_batch_autotune times candidate launch shapes on the batch itself and
  * keeps the fastest: workgroup width x resident workgroups per CU, and for
  * the staged form also the offsets per staged group (2 or 4).
@@ -259,7 +260,8 @@ int psm_count_pairs(psm_ctx* ctx, const psm_survey* survey, const float* vel,
  * rows_per_pass = aperture row offsets summed per launch (0 = the whole
  * aperture in one launch; smaller values bound each launch's cache working
  * set to a few aperture rows, later launches accumulate; -1, the default =
- * automatic: 8 for FP32 traces, one launch for FP16/BF16 traces).  Results
+ * automatic: one launch for the staged form and for FP16/BF16 gathers, 8
+ * for FP32 gathers).  Results
  * do not depend on the shape beyond FP32 summation order. */
 int psm_set_tuning(psm_ctx* ctx, int waves_per_block, int rows_per_pass);
 int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
@@ -268,11 +270,13 @@ int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
  * each aperture trace's time window in LDS once per workgroup of output
  * crosslines (waves_per_block 4 or 8) and gathers from there; on = 0 gathers
  * every sample through the vector memory path; on = -1 (the default) stages
- * FP32 traces with 8 output crosslines per workgroup and 4 crossline offsets
- * per staged group in one launch, and gathers FP16/BF16 traces.  steps (2 or
- * 4) = crossline offsets per staged group when on = 1.  With FP32 traces the
- * two rows +db, -db of each row pair are staged as one window of summed
- * pairs (they share every step's travel time and weight).  The staged form
+ * with 8 output crosslines per workgroup and 4 crossline offsets per staged
+ * group in one launch.  steps (2 or 4) = crossline offsets per staged group
+ * when on = 1.  The two rows +db, -db of each row pair are staged as one
+ * window of summed FP32 pairs (they share every step's travel time and
+ * weight); FP16/BF16 pairs are widened and multiplied by their trace's scale
+ * as they are staged, so the staged steps are FP32 for every storage form.
+ * The staged form
  * applies to whole-volume calls and per-inline calls with contiguous rows;
  * other calls use the gather form.  The switch also selects the adjoint's
  * form (psm_demigrate_volume): staged (LDS accumulator windows, one per

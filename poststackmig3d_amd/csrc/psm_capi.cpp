@@ -897,8 +897,9 @@ int batch_capture(psm_batch* b) {
 // Builds the work queue for workgroups of wpb waves (latest t0 tiles of all
 // volumes first), uploads it and captures the graph.  The kernel form follows
 // the context's staging switch as psm_migrate_volume does: the staged form
-// (workgroups of kAutoStageWaves crosslines; wpb only validated) for FP32
-// traces by default or for any storage when forced on, else the gather form.
+// (workgroups of 4 or 8 output crosslines) by default or when forced on, the
+// gather form when switched off or when a row of the pair layout exceeds
+// kLdsMaxRowBytes.
 // stage_g: crossline offsets per staged group (2 or 4); <= 0: the context's
 // setting when staging is forced on, else kAutoStageSteps.
 int batch_set_shape(psm_batch* b, int wpb, int blocks_per_cu, int stage_g = 0) {
@@ -906,19 +907,17 @@ int batch_set_shape(psm_batch* b, int wpb, int blocks_per_cu, int stage_g = 0) {
   if (wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8)
     return fail(c, PSM_ERR_ARG, "batch waves_per_block must be 1, 2, 4 or 8");
   if (blocks_per_cu < 0 || blocks_per_cu > 64) return fail(c, PSM_ERR_ARG, "bad blocks_per_cu");
-  b->staged = c->tune.staging > 0 || (c->tune.staging < 0 && b->storage == PSM_STORE_F32);
+  b->staged = c->tune.staging != 0;
   // the staged form needs every item's pair-layout rows within kLdsMaxRowBytes
   for (const psm::MigArgs& a : b->args)
     if ((size_t)(a.n_xd + 1) * a.n_t * (b->storage == PSM_STORE_F32 ? 8 : 4) > psm::kLdsMaxRowBytes)
       b->staged = false;
   if (b->staged) {
-    // FP32 traces: 4 or 8 output crosslines per workgroup, 2 or 4 offsets per
-    // group; FP16/BF16 (staging forced on): 8 and 4
-    const bool f32 = b->storage == PSM_STORE_F32;
-    wpb = (f32 && (wpb == 4 || wpb == 8)) ? wpb : psm::kAutoStageWaves;
+    // 4 or 8 output crosslines per workgroup, 2 or 4 offsets per group
+    wpb = (wpb == 4 || wpb == 8) ? wpb : psm::kAutoStageWaves;
     if (stage_g != 2 && stage_g != 4)
       stage_g = c->tune.staging > 0 ? c->tune.stage_steps : psm::kAutoStageSteps;
-    b->stage_g = f32 ? stage_g : psm::kAutoStageSteps;
+    b->stage_g = stage_g;
   }
   b->wpb = wpb;
   b->blocks_per_cu = blocks_per_cu;
@@ -1108,14 +1107,14 @@ int psm_batch_autotune(psm_batch* b, int* waves_per_block, int* blocks_per_cu, f
   psm_ctx* c = b->ctx;
   PSM_HIP(c, hipSetDevice(c->device));
   // candidate launch shapes: workgroup width (output crosslines) x resident
-  // workgroups per CU, and for the staged form of FP32 traces also the
-  // crossline offsets per staged group (the gather form: 0)
+  // workgroups per CU, and for the staged form also the crossline offsets
+  // per staged group (the gather form: 0)
   const int cand_gather[][3] = {{1, 32, 0}, {2, 16, 0}, {4, 8, 0}, {8, 4, 0}, {2, 8, 0}, {4, 4, 0}};
   const int cand_staged[][3] = {{8, kBatchStagedPerCu, 4}, {8, kBatchStagedPerCu - 1, 4},
                                 {8, kBatchStagedPerCu, 2}, {4, kBatchStaged4PerCu, 4},
                                 {4, kBatchStaged4PerCu, 2}, {4, kBatchStaged4PerCu - 1, 4}};
   const bool staged = b->staged;  // the form batch_set_shape chose (it keeps it per shape)
-  const int n_cand = staged ? (b->storage == PSM_STORE_F32 ? 6 : 2) : 6;
+  const int n_cand = 6;
   float best = -1.0f;
   int bw = 4, bb = 0, bg = 0;
   bool warm = false;

@@ -74,11 +74,12 @@ struct BatchSeg {
 };
 
 // Launch-shape parameters (chosen by the autotuner or the caller).
-// Row offsets per launch chosen by rows_per_pass = -1 for FP32 traces: the
-// FP32 gathers are bound by the cache hierarchy, and passes of 8 aperture rows
-// keep the rows that concurrently running workgroups read within L2 (1.2x on
-// the 401 x 401 x 1001 survey); half-precision traces are VALU-bound and run
-// the whole aperture in one launch.
+// Row offsets per launch chosen by rows_per_pass = -1 for the gather form of
+// FP32 traces: the FP32 gathers are bound by the cache hierarchy, and passes
+// of 8 aperture rows keep the rows that concurrently running workgroups read
+// within L2 (1.2x on the 401 x 401 x 1001 survey); half-precision gathers are
+// VALU-bound and run the whole aperture in one launch, as the staged form
+// does.
 constexpr int kAutoRowsF32 = 8;
 
 struct Tuning {
@@ -87,17 +88,16 @@ struct Tuning {
                              // -1 = automatic (kAutoRowsF32 for FP32 traces, else 0)
   int staging = -1;          // forward migration: 1 = aperture windows staged in LDS
                              // (workgroups of 4 or 8 waves, contiguous rows), 0 = gathers,
-                             // -1 = automatic (kAutoStaging for FP32 traces, else gathers)
+                             // -1 = automatic (staged with kAutoStageWaves x kAutoStageSteps)
   int stage_steps = 4;       // crossline offsets per staged group (2 or 4)
 };
 
-// Automatic forward-kernel choice for FP32 traces: windows staged in LDS by
-// workgroups of 8 output crosslines, 4 crossline offsets per staged group
-// (with both rows of a row pair summed into one window: 467 ms on the
-// 401 x 401 x 1001 survey against 517 ms with 2 offsets), the whole aperture
-// in one launch (this beats the gather form, which is bound by the
-// texture-address rate); half-precision traces are VALU-bound in either form
-// and use the gather form.
+// Automatic forward-kernel choice: windows staged in LDS by workgroups of 8
+// output crosslines, 4 crossline offsets per staged group (both rows of a row
+// pair summed into one FP32 window: 372 ms on the 401 x 401 x 1001 survey
+// against 442 ms with 2 offsets), the whole aperture in one launch.  This
+// beats the gather form, which is bound by the texture-address rate, for
+// every storage form (FP16 traces: 387 against 468 ms).
 constexpr int kAutoStageWaves = 8;
 constexpr int kAutoStageSteps = 4;
 

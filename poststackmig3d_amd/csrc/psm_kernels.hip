@@ -155,7 +155,6 @@ template <>
 struct Store<PSM_STORE_F16> {
   using Raw = unsigned;
   static constexpr int kBytes = 4;
-  static constexpr int kShift = 2;  // log2 kBytes
   static constexpr bool kScaled = true;
   __device__ static Raw load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
@@ -172,7 +171,6 @@ template <>
 struct Store<PSM_STORE_BF16> {
   using Raw = unsigned;
   static constexpr int kBytes = 4;
-  static constexpr int kShift = 2;  // log2 kBytes
   static constexpr bool kScaled = true;
   __device__ static Raw load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
@@ -706,21 +704,24 @@ struct StageTab {
 template <int ST, int W, int G, int MODE = kForward>
 struct LdsPlan {
   static constexpr bool kAdj = MODE == kAdjoint;
-  // FP32 forward: the rows +db and -db of a row pair share every step's tau,
-  // i, f, weight and mask, so a column's windows of the two rows are staged
-  // as one window of summed pairs (x_p[i] + x_m[i], d_p[i] + d_m[i]) -- one
-  // ring row, half the LDS reads and interpolations per step.  The adjoint
-  // likewise adds the same terms to both rows, so one accumulator window per
-  // column is flushed into both rows.  The per-trace scaled half-precision
-  // forward keeps one ring row per row.
-  static constexpr bool kSumRows = !kAdj && ST == PSM_STORE_F32;
-  static constexpr int kRingRows = (kSumRows || kAdj) ? 1 : 2;
-  // FP32 forward: a window may run past the trace end -- its pairs past
-  // n_t - 1 are zeros in the slot -- so the steps read pair i without
-  // clamping it (two 8-byte pairs per 16-byte staging lane)
+  // Forward: the rows +db and -db of a row pair share every step's tau, i,
+  // f, weight and mask, so a column's windows of the two rows are staged as
+  // one window of summed FP32 pairs (x_p[i] + x_m[i], d_p[i] + d_m[i]) --
+  // one ring row, half the LDS reads and interpolations per step.  FP16 /
+  // BF16 rows are widened and multiplied by their trace's scale as they are
+  // staged (s_p x_p[i] + s_m x_m[i], ...), so the steps are the FP32 steps.
+  // The adjoint likewise adds the same terms to both rows, so one
+  // accumulator window per column is flushed into both rows.
+  static constexpr bool kSumRows = !kAdj;
+  static constexpr int kRingRows = 1;
+  // forward: a window may run past the trace end -- its pairs past n_t - 1
+  // are zeros in the slot -- so the steps read pair i without clamping it
+  // (two pairs per staging lane: 16 bytes of FP32 pairs, 8 of half pairs)
   static constexpr bool kOpenEnd = kSumRows;
-  static constexpr int kB = kAdj ? 8 : Store<ST>::kBytes;  // bytes per pair
-  static constexpr int kSlot = kWinPairs * kB;           // bytes of one window
+  static constexpr int kB = 8;                      // LDS bytes per pair (FP32 pair or (A, B))
+  static constexpr int kHB = Store<ST>::kBytes;     // bytes per stored pair in the cube
+  static constexpr int kHShift = kHB == 8 ? 0 : 1;  // LDS bytes -> cube bytes
+  static constexpr int kSlot = kWinPairs * kB;      // bytes of one window
   // adjoint: copy u of a column's window takes the adds of step u of a group
   // (a column is touched by a different wave at each step, so waves that
   // drift apart within a group never share a copy)
@@ -771,7 +772,6 @@ template <int ST, int W, int G, bool AMP, int MODE = kForward>
 __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int tile, int io,
                                             int j0, char* smem) {
   using S = Store<ST>;
-  using Raw = typename S::Raw;
   using PL = LdsPlan<ST, W, G, MODE>;
   constexpr int kB = PL::kB;
   const int w = uniform_i((int)(threadIdx.x / kWave));
@@ -838,7 +838,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
     const int c = tk.c;
     const int nt1 = a.n_t - 1;
     const int xd_lo = a.xd_off, xd_hi = a.xd_off + a.n_xd;
-    const int tb = a.n_t * kB;
+    const int tb = a.n_t * PL::kHB;  // bytes of one stored trace
     const size_t row_bytes = (size_t)(a.n_xd + 1) * tb;
     const char* const base = reinterpret_cast<const char*>(a.xp);
     const char* const zero_row = base + (size_t)a.n_rows * row_bytes;
@@ -1179,50 +1179,98 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         }
       } else if (!fallback) {
         // ---- staging jobs: job j of a set of columns [oa, oa + nc) is
-        // (column oa + j / (2 nr), side (j / nr) % 2, ring row j % nr), nr =
-        // rows of the pair; wave w takes jobs w, w + W, ...  A job loads the
-        // column's window of one row, 16 bytes per lane: the first columns of
-        // a row straight into LDS, later ones into registers that are written
-        // to the column's ring slot at the end of a group, so LDS reads never
-        // wait on loads in flight.  Pairs past the trace end are loaded out
-        // of range; columns outside the data are zeros (out of range in the
-        // first fill, the row's zero trace afterwards).
+        // (column oa + j / 2, side j % 2); wave w takes jobs w, w + W, ...
+        // A job loads the column's window of both rows of the pair, two pairs
+        // per lane (16 bytes of FP32 pairs, 8 of half pairs, per row), and
+        // stores their sum: in the first fill of a row at once, later through
+        // registers that are written to the column's ring slot at the end of
+        // a group, so LDS reads never wait on loads in flight.  Pairs past
+        // the trace end are loaded out of range; columns outside the data
+        // are zeros (out of range in the first fill, the row's zero trace
+        // afterwards).
         using V4 = unsigned __attribute__((ext_vector_type(4)));
+        using V2 = unsigned __attribute__((ext_vector_type(2)));
         using F4 = float __attribute__((ext_vector_type(4)));
+        using JV = std::conditional_t<S::kScaled, V2, V4>;  // one row of a job, per lane
         const int lb = lane * 16;
         const int lbs = lb < PL::kSlot ? lb : 0x7ff00000;  // lanes past the slot load nothing
         const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<char*>(base + (size_t)r0 * row_bytes), 0, (int)row_bytes, kRsrcWord3);
         const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<char*>(base + (size_t)rm * row_bytes), 0, (int)row_bytes, kRsrcWord3);
-        // A job's source: column o, side, ring row rr of the job index j
-        // within the columns [oa, oa + nc).
+        // scale tables of the staged rows (F16 / BF16; entry n_xd is the
+        // row's zero trace)
+        const const_f32_ptr sc0 = as_const(a.scale + (size_t)r0 * row_tr);
+        const const_f32_ptr sc1 = as_const(a.scale + (size_t)rm * row_tr);
+        auto load_row = [&](__amdgpu_buffer_rsrc_t r, int voff, int soff) -> JV {
+          if constexpr (S::kScaled)
+            return __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+          else
+            return __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        };
+        // the two stored pairs of one row as FP32 pairs (sample, difference),
+        // unscaled
+        auto unpack = [&](JV v) -> F4 {
+          if constexpr (ST == PSM_STORE_F32) {
+            return __builtin_bit_cast(F4, v);
+          } else if constexpr (ST == PSM_STORE_F16) {
+            return F4{(float)__builtin_bit_cast(_Float16, (unsigned short)(v.x & 0xffffu)),
+                      (float)__builtin_bit_cast(_Float16, (unsigned short)(v.x >> 16)),
+                      (float)__builtin_bit_cast(_Float16, (unsigned short)(v.y & 0xffffu)),
+                      (float)__builtin_bit_cast(_Float16, (unsigned short)(v.y >> 16))};
+          } else {
+            return F4{__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xffff0000u),
+                      __uint_as_float(v.y << 16), __uint_as_float(v.y & 0xffff0000u)};
+          }
+        };
+        // one row's pairs times its trace's scale s (F16 / BF16; FP32 rows
+        // are not scaled).  fma(x, s, 0) is x s but for the sign of an exact
+        // zero (no matter in a sum), and lets an F16 x be widened and scaled
+        // by one v_fma_mix_f32.
+        auto widen = [&](JV v, float s) -> F4 {
+          if constexpr (S::kScaled) {
+            const F4 x = unpack(v);
+            return F4{__builtin_fmaf(x.x, s, 0.0f), __builtin_fmaf(x.y, s, 0.0f),
+                      __builtin_fmaf(x.z, s, 0.0f), __builtin_fmaf(x.w, s, 0.0f)};
+          }
+          return unpack(v);
+        };
+        // acc + the row's pairs (times s for F16 / BF16)
+        auto widen_add = [&](JV v, float s, F4 acc) -> F4 {
+          const F4 x = unpack(v);
+          if constexpr (S::kScaled)
+            return F4{__builtin_fmaf(x.x, s, acc.x), __builtin_fmaf(x.y, s, acc.y),
+                      __builtin_fmaf(x.z, s, acc.z), __builtin_fmaf(x.w, s, acc.w)};
+          return acc + x;
+        };
+        // A job's source: column o, side of the job index j within the
+        // columns [oa, oa + nc).
         // (one return statement: a lambda returning this struct on two paths
         // made the compiler copy the kernel arguments to scratch)
-        // log2 of the jobs per column (summed rows: one job loads both rows)
-        const int jsh = (two && !PL::kSumRows) ? 2 : 1;
         auto job_at = [&](int oa, int nc, int j) {
-          return StageJob{oa + (j >> jsh), (j >> (jsh - 1)) & 1, j & (jsh - 1), j < (nc << jsh)};
+          return StageJob{oa + (j >> 1), j & 1, 0, j < 2 * nc};
         };
         // buffer offsets of a job whose table entry is m (meta of its column)
-        // (mw = the column's window bytes: lanes past pairs lo .. hi load
-        // nothing)
-        auto job_src = [&](const StageJob& jb, int m, int mw, int& voff, int& soff) {
+        // and its trace's index in the scale table (mw = the column's window
+        // bytes in LDS: lanes past pairs lo .. hi load nothing)
+        auto job_src = [&](const StageJob& jb, int m, int mw, int& voff, int& soff, int& si) {
           const int k = jb.side == 0 ? kbase + jb.o : kbase + (W - 1) - jb.o;
           const bool data = k >= xd_lo && k < xd_hi;
           const int lo = uniform_i((PL::slot(jb.side, jb.o) - m) / kB);
           soff = data ? (k - xd_lo) * tb : 0;
+          si = data ? k - xd_lo : a.n_xd;
           const bool zero = !data || lb >= mw;
-          voff = zero ? 0x7ffffff0 : lo * kB + lb;
+          voff = zero ? 0x7ffffff0 : (lo * kB + lb) >> PL::kHShift;
         };
-        auto job_dst = [&](const StageJob& jb) { return PL::slot(jb.side, jb.o) + jb.rr * PL::kRow; };
+        auto job_dst = [&](const StageJob& jb) { return PL::slot(jb.side, jb.o); };
         auto tab = [&](int side, int o) { return meta[side * PL::kCols + o]; };
         auto wtab = [&](int side, int o) {
           return reinterpret_cast<const int*>(smem + PL::kWid)[side * PL::kCols + o];
         };
-        {  // the first G + W - 1 columns, loaded straight into LDS (no
-           // registers held); complete before the barrier
-          constexpr int kFill = (2 * PL::kRingRows * (G + W - 1) + W - 1) / W;
+        {  // the first G + W - 1 columns (no registers held across the
+           // barrier): FP32 single rows straight into LDS, summed or scaled
+           // rows through registers; complete before the barrier
+          constexpr int kFill = (2 * (G + W - 1) + W - 1) / W;
           const int nc = min(G + W - 1, nO);
           int m[kFill], mw[kFill];
 #pragma unroll
@@ -1235,61 +1283,50 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           for (int t = 0; t < kFill; ++t) {
             const StageJob jb = job_at(0, nc, w + t * W);
             if (jb.ok && lb < PL::kSlot) {
-              int voff, soff;
-              job_src(jb, m[t], mw[t], voff, soff);
-              if (PL::kSumRows && two) {
-                // both rows through registers, their sum into the slot
-                const F4 x0 = __builtin_bit_cast(
-                    F4, __builtin_amdgcn_raw_buffer_load_b128(rs0, voff, soff, 0));
-                const F4 x1 = __builtin_bit_cast(
-                    F4, __builtin_amdgcn_raw_buffer_load_b128(rs1, voff, soff, 0));
-                *reinterpret_cast<F4*>(smem + job_dst(jb) + lb) = x0 + x1;
+              int voff, soff, si;
+              job_src(jb, m[t], mw[t], voff, soff, si);
+              if (S::kScaled || two) {
+                F4 x = widen(load_row(rs0, voff, soff), S::kScaled ? sc0[si] : 1.0f);
+                if (two) x = widen_add(load_row(rs1, voff, soff), S::kScaled ? sc1[si] : 1.0f, x);
+                *reinterpret_cast<F4*>(smem + job_dst(jb) + lb) = x;
               } else {
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    jb.rr == 0 ? rs0 : rs1,
-                    (__attribute__((address_space(3))) void*)(smem + job_dst(jb)), 16, voff, soff,
-                    0, 0);
+                    rs0, (__attribute__((address_space(3))) void*)(smem + job_dst(jb)), 16, voff,
+                    soff, 0, 0);
               }
             }
           }
           __builtin_amdgcn_s_waitcnt(0);
         }
-        // scale tables of the staged rows (F16 / BF16)
-        const const_f32_ptr sc0 = as_const(a.scale + (size_t)r0 * row_tr);
-        const const_f32_ptr sc1 = as_const(a.scale + (size_t)rm * row_tr);
         // Table values of one group, read one group ahead so that their LDS
         // latency overlaps the stores and the barrier: r^2, the two column
         // addresses of each step, and the table entries of the staging jobs
         // for the group after it.
         // staging jobs per wave per group, at most
-        constexpr int kJobs = (2 * PL::kRingRows * G + W - 1) / W;
+        constexpr int kJobs = (2 * G + W - 1) / W;
         // This wave's jobs in every full group of the row: job q is column
-        // oa + jc of side js, ring row jr, if the wave has that job at all
-        // (oa = the group's first new column), so the jobs' state advances
-        // by fixed steps from group to group and is kept in scalar
-        // registers: jo = column offset of the next load, jsl = its ring slot
-        // (+ ring row offset), jkd = its data crossline (buffer crossline -
-        // xd_lo), jso = jkd * tb.  A column outside the data loads the row's
-        // zero trace (its last); a job past the row's columns (or a wave
-        // without the job) loads a column like any other and stores into the
-        // spare slot.
+        // oa + jc of side js if the wave has that job at all (oa = the
+        // group's first new column), so the jobs' state advances by fixed
+        // steps from group to group and is kept in scalar registers: jo =
+        // column offset of the next load, jsl = its ring slot, jkd = its data
+        // crossline (buffer crossline - xd_lo), jso = jkd * tb.  A column
+        // outside the data loads the row's zero trace (its last); a job past
+        // the row's columns (or a wave without the job) loads a column like
+        // any other and stores into the spare slot.
         int js[kJobs], jo[kJobs], jsl[kJobs], jkd[kJobs], jso[kJobs];
-        int jlim[kJobs], jslend[kJobs], jdkd[kJobs], jlbs[kJobs];
-        __amdgpu_buffer_rsrc_t jrs[kJobs];
+        int jlim[kJobs], jslend[kJobs], jdkd[kJobs];
 #pragma unroll
         for (int q = 0; q < kJobs; ++q) {
           const StageJob jb = job_at(0, G, w + q * W);
-          const int jc = uniform_i(jb.o), jr = uniform_i(jb.rr);
+          const int jc = uniform_i(jb.o);
           js[q] = uniform_i(jb.side);
           jo[q] = G + W - 1 + jc;
           jlim[q] = jb.ok ? nO : -1;
-          jsl[q] = PL::slot(js[q], jo[q]) + jr * PL::kRow;
-          jslend[q] = js[q] * PL::kSide + jr * PL::kRow + PL::kRing * PL::kSlot;
+          jsl[q] = PL::slot(js[q], jo[q]);
+          jslend[q] = js[q] * PL::kSide + PL::kRing * PL::kSlot;
           jkd[q] = (js[q] == 0 ? kbase + jo[q] : kbase + (W - 1) - jo[q]) - xd_lo;
           jdkd[q] = js[q] == 0 ? G : -G;
           jso[q] = jkd[q] * tb;
-          jrs[q] = jr == 0 ? rs0 : rs1;
-          jlbs[q] = lbs - jr * PL::kRow;  // load offsets are relative to the side's slot
         }
         using Tab = StageTab<G, kJobs>;
         auto read_tab = [&](int dk0, Tab& t) {
@@ -1313,23 +1350,20 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // to kmax_wg, and a step past a wave's own last offset adds only masked
         // terms or exact zeros (pairs past the trace end).  FIRST: the group
         // holds dk = 0, whose -dk column is the +dk column and is left out.
-        auto compute = [&](auto nr_c, auto full_c, auto first_c, auto mask_c, int dk0, int n,
-                           const Tab& t) {
-          constexpr int NR = decltype(nr_c)::value;
+        auto compute = [&](auto full_c, auto first_c, auto mask_c, int dk0, int n, const Tab& t) {
+          using LS = Store<PSM_STORE_F32>;  // the windows hold FP32 pairs
           constexpr bool MASK = decltype(mask_c)::value;  // false: every lane is inside
           constexpr bool FULL = decltype(full_c)::value;
           constexpr bool FIRST = decltype(first_c)::value;
           struct Stp {
             float f, wt;
             bool m;
-            Raw x[2 * NR];
-            float sc[2 * NR];
+            LS::Raw x[2];
           };
           Stp st[G];
 #pragma unroll
           for (int u = 0; u < G; ++u) {
             if (FULL || u < n) {
-              const int dk = dk0 + u;
               const bool one = FIRST && u == 0;  // dk = 0: one column
               const float r2 = t.r2[u];
               const TauW tw = tau_weight(p, r2, AMP);
@@ -1337,57 +1371,26 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
               // both column addresses as (i << log2 kB) + base, one
               // instruction each (the compiler shares the shift otherwise)
               // (a lane past the trace end reads a zero pair of the window)
-              const int i = PL::kOpenEnd ? (int)q : min((int)q, nt1);
+              const int i = (int)q;
               int ar, al;
-              asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(ar) : "v"(i), "i"(S::kShift), "v"(t.bR[u]));
-              asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(al) : "v"(i), "i"(S::kShift), "v"(t.bL[u]));
-              const char* xr = smem + ar;
-              const char* xl = smem + al;
-              st[u].x[0] = *reinterpret_cast<const Raw*>(xr);
-              st[u].x[1] = one ? Raw{} : *reinterpret_cast<const Raw*>(xl);
-              if constexpr (NR == 2) {
-                st[u].x[2] = *reinterpret_cast<const Raw*>(xr + PL::kRow);
-                st[u].x[3] = one ? Raw{} : *reinterpret_cast<const Raw*>(xl + PL::kRow);
-              }
-              if constexpr (S::kScaled) {
-                int cR = kj + dk - xd_lo, cL = kj - dk - xd_lo;
-                cR = (unsigned)cR < (unsigned)a.n_xd ? cR : a.n_xd;
-                cL = (!one && (unsigned)cL < (unsigned)a.n_xd) ? cL : a.n_xd;
-                st[u].sc[0] = sc0[cR];
-                st[u].sc[1] = sc0[cL];
-                if constexpr (NR == 2) {
-                  st[u].sc[2] = sc1[cR];
-                  st[u].sc[3] = sc1[cL];
-                }
-              }
+              asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(ar) : "v"(i), "i"(LS::kShift), "v"(t.bR[u]));
+              asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(al) : "v"(i), "i"(LS::kShift), "v"(t.bL[u]));
+              st[u].x[0] = LS::load_ptr(smem + ar);
+              st[u].x[1] = one ? LS::Raw{} : LS::load_ptr(smem + al);
               st[u].f = __builtin_amdgcn_fractf(q);
               st[u].wt = tw.w;
               if constexpr (MASK) st[u].m = r2 <= p.R2;
             }
           }
-          // slot order (row, column) = (+db,+dk), (+db,-dk), (-db,+dk), (-db,-dk)
+          // each window holds the sum of the rows +db and -db: columns +dk
+          // and -dk
 #pragma unroll
           for (int u = 0; u < G; ++u) {
             if (FULL || u < n) {
               const Stp& q = st[u];
               const bool one = FIRST && u == 0;
-              float sum;
-              if constexpr (S::kScaled) {
-                sum = S::lerp(q.x[0], q.f) * q.sc[0];
-                if (!one) sum = __builtin_fmaf(S::lerp(q.x[1], q.f), q.sc[1], sum);
-                if constexpr (NR == 2) {
-                  sum = __builtin_fmaf(S::lerp(q.x[2], q.f), q.sc[2], sum);
-                  if (!one) sum = __builtin_fmaf(S::lerp(q.x[3], q.f), q.sc[3], sum);
-                }
-              } else if (one) {
-                sum = S::lerp(q.x[0], q.f);
-                if constexpr (NR == 2) sum = sum + S::lerp(q.x[2], q.f);
-              } else if constexpr (NR == 2) {
-                sum = (S::lerp(q.x[0], q.f) + S::lerp(q.x[1], q.f)) +
-                      (S::lerp(q.x[2], q.f) + S::lerp(q.x[3], q.f));
-              } else {
-                sum = S::lerp(q.x[0], q.f) + S::lerp(q.x[1], q.f);
-              }
+              float sum = LS::lerp(q.x[0], q.f);
+              if (!one) sum = sum + LS::lerp(q.x[1], q.f);
               if constexpr (MASK) sum = q.m ? sum : 0.0f;
               acc = __builtin_fmaf(sum, q.wt, acc);
             }
@@ -1404,10 +1407,13 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // of their reads: the loads issued at the start of group dk0 are held
         // in registers through groups dk0 and dk0 + G, stored at the end of
         // group dk0 + G and read from group dk0 + 2G on.
-        // (summed rows: the second row's loads follow the first's, v[kJobs + q])
-        using Jobs = V4[PL::kSumRows ? 2 * kJobs : kJobs];
+        // A register set: the loads of this wave's jobs (v[q] the first row,
+        // v[kJobs + q] the second row of a pair), their destination slots and
+        // their scales (F16 / BF16).
+        using Jobs = JV[2 * kJobs];
         using Dsts = int[kJobs];
-        auto issue = [&](auto nr_c, const int* m, const int* mw, Jobs& v, Dsts& dst) {
+        using Scls = float[2 * kJobs];
+        auto issue = [&](auto nr_c, const int* m, const int* mw, Jobs& v, Dsts& dst, Scls& sc) {
           constexpr int NR = decltype(nr_c)::value;
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
@@ -1415,18 +1421,21 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
             // = slot - table entry m); lanes whose pairs all lie past the
             // trace end load out of range (no read reaches those slot
             // entries: reads stop at hi <= n_t - 1), the lane holding pair
-            // n_t - 1 reads at most 12 bytes into the next trace of the row
-            // (every row ends with a zero trace)
+            // n_t - 1 ends its load there (lo makes n_t - 1 - lo odd)
             const bool data = (unsigned)jkd[q] < (unsigned)a.n_xd;
             dst[q] = jo[q] < jlim[q] ? jsl[q] : PL::kSpare;
-            const int voff = jsl[q] + jlbs[q] - m[q];  // lo * kB + lane * 16
+            const int voff = jsl[q] + lbs - m[q];  // lo * kB + lane * 16 (LDS bytes)
             // (lanes past the window's pairs lo .. hi -- hi <= n_t - 1 --
             // load nothing)
-            const int vo = lb < mw[q] ? voff : 0x7ff00000;
+            const int vo = lb < mw[q] ? voff >> PL::kHShift : 0x7ff00000;
             const int so = data ? jso[q] : a.n_xd * tb;
-            v[q] = __builtin_amdgcn_raw_buffer_load_b128(jrs[q], vo, so, 0);
-            if constexpr (PL::kSumRows && NR == 2)
-              v[PL::kSumRows ? kJobs + q : q] = __builtin_amdgcn_raw_buffer_load_b128(rs1, vo, so, 0);
+            v[q] = load_row(rs0, vo, so);
+            if constexpr (NR == 2) v[kJobs + q] = load_row(rs1, vo, so);
+            if constexpr (S::kScaled) {
+              const int si = data ? jkd[q] : a.n_xd;
+              sc[q] = sc0[si];
+              if constexpr (NR == 2) sc[kJobs + q] = sc1[si];
+            }
             jo[q] += G;
             jkd[q] += jdkd[q];
             jso[q] += jdkd[q] * tb;
@@ -1438,17 +1447,15 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // group, read the next group's tables, store the loads issued one
         // group earlier (vs) and make them visible.
         auto run_group = [&](auto nr_c, auto full_c, auto first_c, auto mask_c, int dk0, Tab& t,
-                             Jobs& vs, Dsts& ds, Jobs& vi, Dsts& di) {
+                             Jobs& vs, Dsts& ds, Scls& ss, Jobs& vi, Dsts& di, Scls& si) {
           // (wave priority: the loads go out first and the group's tail --
           // stores and barrier -- is not held up behind other waves'
           // sums; 0.75 % on the production survey)
           __builtin_amdgcn_s_setprio(1);
-          issue(nr_c, t.mj, t.mw, vi, di);
+          issue(nr_c, t.mj, t.mw, vi, di, si);
           __builtin_amdgcn_s_setprio(0);
-          // (summed rows: one ring row holds both rows of the pair)
           constexpr int NR = decltype(nr_c)::value;
-          compute(std::integral_constant<int, PL::kSumRows ? 1 : NR>{}, full_c, first_c, mask_c,
-                  dk0, min(G, kmax_wg + 1 - dk0), t);
+          compute(full_c, first_c, mask_c, dk0, min(G, kmax_wg + 1 - dk0), t);
           read_tab(dk0 + G, t);
           // The stores wait on the loads, so the group's sum must be complete
           // before them: the empty asm takes the accumulator as an input, and
@@ -1461,12 +1468,10 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
             if (lb < PL::kSlot) {
-              if constexpr (PL::kSumRows && NR == 2)
-                *reinterpret_cast<F4*>(smem + ds[q] + lb) =
-                    __builtin_bit_cast(F4, vs[q]) +
-                    __builtin_bit_cast(F4, vs[PL::kSumRows ? kJobs + q : q]);
-              else
-                *reinterpret_cast<V4*>(smem + ds[q] + lb) = vs[q];
+              F4 x = widen(vs[q], S::kScaled ? ss[q] : 1.0f);
+              if constexpr (NR == 2)
+                x = widen_add(vs[kJobs + q], S::kScaled ? ss[kJobs + q] : 1.0f, x);
+              *reinterpret_cast<F4*>(smem + ds[q] + lb) = x;
             }
           }
           __syncthreads();  // the next group's windows are visible
@@ -1477,6 +1482,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           using F = std::integral_constant<bool, false>;
           Jobs va, vb;
           Dsts da, db;
+          Scls sa, sb;
           {  // the loads stored at the end of group 0
             int m0[kJobs], w0[kJobs];
 #pragma unroll
@@ -1484,32 +1490,32 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
               m0[q] = tab(js[q], jo[q]);
               w0[q] = wtab(js[q], jo[q]);
             }
-            issue(nr_c, m0, w0, va, da);
+            issue(nr_c, m0, w0, va, da, sa);
           }
           Tab t;
           read_tab(0, t);
           __syncthreads();  // the first windows are visible
           if (kmax_wg + 1 >= G)
-            run_group(nr_c, T{}, T{}, T{}, 0, t, va, da, vb, db);
+            run_group(nr_c, T{}, T{}, T{}, 0, t, va, da, sa, vb, db, sb);
           else
-            run_group(nr_c, F{}, T{}, T{}, 0, t, va, da, vb, db);
+            run_group(nr_c, F{}, T{}, T{}, 0, t, va, da, sa, vb, db, sb);
           // groups alternate the two register sets (unrolled by two); pairs
           // of groups inside every lane's aperture (dk <= kin) skip the mask
           // (the barriers are the same on both paths)
           int dk0 = G;
           for (; dk0 + 2 * G <= min(kmax_wg, kin) + 1; dk0 += 2 * G) {
-            run_group(nr_c, T{}, F{}, F{}, dk0, t, vb, db, va, da);
-            run_group(nr_c, T{}, F{}, F{}, dk0 + G, t, va, da, vb, db);
+            run_group(nr_c, T{}, F{}, F{}, dk0, t, vb, db, sb, va, da, sa);
+            run_group(nr_c, T{}, F{}, F{}, dk0 + G, t, va, da, sa, vb, db, sb);
           }
           for (; dk0 + 2 * G <= kmax_wg + 1; dk0 += 2 * G) {
-            run_group(nr_c, T{}, F{}, T{}, dk0, t, vb, db, va, da);
-            run_group(nr_c, T{}, F{}, T{}, dk0 + G, t, va, da, vb, db);
+            run_group(nr_c, T{}, F{}, T{}, dk0, t, vb, db, sb, va, da, sa);
+            run_group(nr_c, T{}, F{}, T{}, dk0 + G, t, va, da, sa, vb, db, sb);
           }
           if (dk0 + G <= kmax_wg + 1) {
-            run_group(nr_c, T{}, F{}, T{}, dk0, t, vb, db, va, da);
-            if (dk0 + G <= kmax_wg) run_group(nr_c, F{}, F{}, T{}, dk0 + G, t, va, da, vb, db);
+            run_group(nr_c, T{}, F{}, T{}, dk0, t, vb, db, sb, va, da, sa);
+            if (dk0 + G <= kmax_wg) run_group(nr_c, F{}, F{}, T{}, dk0 + G, t, va, da, sa, vb, db, sb);
           } else if (dk0 <= kmax_wg) {
-            run_group(nr_c, F{}, F{}, T{}, dk0, t, vb, db, va, da);
+            run_group(nr_c, F{}, F{}, T{}, dk0, t, vb, db, sb, va, da, sa);
           }
         };
         if (two)
@@ -1530,14 +1536,19 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
     if constexpr (MODE == kForward && AMP) acc *= lane_iv2;
   }
   if constexpr (MODE == kForward) {
-    if (in_out && active) {
+    // the lane's output sample, formed again from the lane index (which the
+    // aperture loop keeps live) rather than held through the loop
+    int le = lane;
+    asm volatile("" : "+v"(le));
+    const int so = tile * kWave + le;
+    if (in_out && so < a.n_t) {
       float* orow = a.out + ((size_t)io * a.n_out_xl + j) * a.n_t;
       if (!in_buf) {
-        if (!a.accumulate) orow[s] = 0.0f;  // migration_kernel.cpp:79-85
+        if (!a.accumulate) orow[so] = 0.0f;  // migration_kernel.cpp:79-85
       } else if (a.accumulate) {
-        if (valid) orow[s] += acc;
+        if (valid) orow[so] += acc;
       } else {
-        orow[s] = valid ? acc : 0.0f;  // :148-151, :375
+        orow[so] = valid ? acc : 0.0f;  // :148-151, :375
       }
     }
   }
@@ -1923,7 +1934,7 @@ hipError_t launch_migrate(const MigArgs& a0, int mode, hipStream_t stream, const
   // workgroup, rows addressable by a 32-bit buffer size
   const size_t row_bytes = (size_t)(a0.n_xd + 1) * a0.n_t * (a0.storage == PSM_STORE_F32 ? 8 : 4);
   const bool lds_ok = mode == kForward && a0.row_pos == nullptr && row_bytes <= kLdsMaxRowBytes;
-  const bool auto_lds = tune.staging < 0 && a0.storage == PSM_STORE_F32 && lds_ok;
+  const bool auto_lds = tune.staging < 0 && lds_ok;
   const int wpb = auto_lds ? kAutoStageWaves : tune.waves_per_block;
   const int steps = auto_lds ? kAutoStageSteps : tune.stage_steps;
   const bool lds = auto_lds || (tune.staging > 0 && lds_ok && (wpb == 4 || wpb == 8));
@@ -1987,19 +1998,13 @@ hipError_t launch_batch_st(bool staged, int stage_g, const MigArgs* items, const
                            int blocks, hipStream_t stream) {
   const dim3 grid((unsigned)blocks);
   if (staged) {
-    // shapes the batch autotuner chooses from (FP32 traces); other storages
-    // stage with the automatic shape only
-    if constexpr (ST == PSM_STORE_F32) {
-      if (wpb == 4)
-        (stage_g == 2 ? launch_batch_lds<ST, 4, 2> : launch_batch_lds<ST, 4, 4>)(
-            items, segs, n_segs, counters, totals, blocks, stream);
-      else
-        (stage_g == 2 ? launch_batch_lds<ST, 8, 2> : launch_batch_lds<ST, 8, 4>)(
-            items, segs, n_segs, counters, totals, blocks, stream);
-    } else {
-      launch_batch_lds<ST, kAutoStageWaves, kAutoStageSteps>(items, segs, n_segs, counters,
-                                                             totals, blocks, stream);
-    }
+    // the shapes the batch autotuner chooses from
+    if (wpb == 4)
+      (stage_g == 2 ? launch_batch_lds<ST, 4, 2> : launch_batch_lds<ST, 4, 4>)(
+          items, segs, n_segs, counters, totals, blocks, stream);
+    else
+      (stage_g == 2 ? launch_batch_lds<ST, 8, 2> : launch_batch_lds<ST, 8, 4>)(
+          items, segs, n_segs, counters, totals, blocks, stream);
     return hipGetLastError();
   }
   switch (wpb) {

@@ -4,8 +4,8 @@ the single-volume migration of that item at the same velocity scale by the
 same form of the forward kernel in one launch (the same per-workgroup
 arithmetic and summation order, so bit for bit), on every replay of the graph
 with eager work in between, and after autotuning.  The batch takes its form
-from the context's staging switch like psm_migrate_volume: staged for FP32
-traces by default, gather for FP16, gather for FP32 with staging off."""
+from the context's staging switch like psm_migrate_volume: staged by
+default for every storage form, gather with staging off."""
 import numpy as np
 import pytest
 
@@ -58,12 +58,13 @@ def _reference(items, storage, wpb, staged):
 
 
 @pytest.mark.parametrize("storage,staging",
-                         [("f32", None), ("f16", None), ("f32", False), ("f16", True)])
+                         [("f32", None), ("f16", None), ("bf16", None), ("f32", False),
+                          ("f16", False)])
 def test_batch_matches_single_volumes(storage, staging):
     import torch
     from poststackmig3d_amd.batch import Batch
     items = _items(torch, storage)
-    staged = staging is True or (storage == "f32" and staging is None)
+    staged = staging is not False
     refs = _reference(items, storage, 4, staged)
     bctx = Context(0)
     bctx.set_staging(staging)
@@ -84,9 +85,9 @@ def test_batch_matches_single_volumes(storage, staging):
         is_staged, w2, bpc2, steps = b.shape()
         assert is_staged == staged and w2 == w and bpc2 == bpc
         assert steps in ((2, 4) if staged else (0,))
-        assert not staged or w in ((4, 8) if storage == "f32" else (8,))
+        assert not staged or w in (4, 8)
         # every staged shape gives the same result: the forced shapes of the tuner
-        if staged and storage == "f32":
+        if staged:
             for ww, gg in ((4, 2), (8, 2), (4, 4)):
                 b.ctx.set_staging(True, gg)
                 b.set_shape(ww, 0)
