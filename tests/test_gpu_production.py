@@ -9,11 +9,15 @@ main.cpp's loop (oracle/_ref, else the oracle restatement): 8 output inlines
 padding with the first real crosslines [0, 48), the centre [206, 254), and
 the last real crosslines with the whole sticky trailing padding [413, 461).
 
-Bars: relative L2 <= 1e-5 over each window and <= 1e-4 per output trace;
-outputs where the reference is exactly zero (leading padded crosslines, the
-t0 = 0 row) must be exactly zero.  Size-independent properties are checked
-on the whole image: finite, padded output inlines bit-equal to the edge
-inlines, leading padded crosslines and the t0 = 0 row zero."""
+Bars: FP32 traces, relative L2 <= 1e-5 over each window and <= 1e-4 per
+output trace; FP16 / BF16 trace storage (per-trace max-abs scaling, FP32
+arithmetic), relative L2 <= 5e-4 / 4e-3 over each window and <= 1.5e-3 / 1e-2
+per output trace against the same FP32 reference, about 2.5x what MI355X gives
+(FP16 <= 2.0e-4 and 5.3e-4, BF16 <= 1.6e-3 and 4.2e-3).  Outputs where the reference is
+exactly zero (leading padded crosslines, the t0 = 0 row) must be exactly zero.
+Size-independent properties are checked on the whole image: finite, padded
+output inlines bit-equal to the edge inlines, leading padded crosslines and
+the t0 = 0 row zero."""
 import numpy as np
 import pytest
 
@@ -41,7 +45,11 @@ def per_trace_rel(got, ref):
     return float((num[~zero] / den[~zero]).max()) if (~zero).any() else 0.0
 
 
-def test_production_survey_against_reference():
+BARS = {"f32": (1e-5, 1e-4), "f16": (5e-4, 1.5e-3), "bf16": (4e-3, 1e-2)}
+
+
+@pytest.fixture(scope="module")
+def survey():
     import torch
     dev = torch.device("cuda:0")
     print("production: synthesising 401x401x1001 on the GPU", flush=True)
@@ -50,32 +58,50 @@ def test_production_survey_against_reference():
     prm = P.SurveyParams.for_cube(SHAPE, **GEO)
     info = P.survey_plan(prm, vel)
     assert (info.n_out_il, info.n_out_xl, info.max_aperture_il) == (461, 461, 157)
-    print("production: migrating the whole volume", flush=True)
-    out = P.migrate_volume(data, vel, prm)
-    img = out.cpu().numpy()
     d_np, v_np = data.cpu().numpy(), vel.cpu().numpy()
-    del out, data, vel
-    assert np.isfinite(img).all()
-    assert np.array_equal(img[:30], np.broadcast_to(img[30], img[:30].shape))
-    assert np.array_equal(img[431:], np.broadcast_to(img[430], img[431:].shape))
-    assert not img[:, :30].any() and not img[:, :, 0].any()
-
     kind = "reference" if O.have_ref() else "oracle"
     fn = O.ref_migrate_sample if O.have_ref() else O.migrate_sample
     rows = np.array(ROWS, np.int32)
+    refs = []
     for xl0, m in WINDOWS:
         print(f"production: {kind} sample rows {ROWS} x crosslines [{xl0}, {xl0 + m})",
               flush=True)
         ref = fn(d_np, v_np, GEO["dt"], GEO["inline_step"], GEO["crossline_step"],
                  GEO["inline_padding"], GEO["crossline_padding"], GEO["angle_aperture"], True,
                  rows, xl0, m)
-        got = img[rows][:, xl0:xl0 + m]
-        err = O.rel_l2(got, ref)
-        worst = per_trace_rel(got, ref)
-        print(f"production window [{xl0}, {xl0 + m}): rel-L2 {err:.3e}, worst trace {worst:.3e}",
-              flush=True)
-        assert err <= 1e-5, err
-        assert worst <= 1e-4, worst
         if xl0 >= 413:
             # the sticky trailing crosslines carry real (nonzero) migrations
             assert np.abs(ref[:, 431 - xl0:]).max() > 0
+        refs.append((xl0, m, ref))
+    del d_np, v_np
+    yield data, vel, prm, rows, refs
+    del data, vel
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_production_survey_against_reference(survey, storage):
+    from poststackmig3d_amd._lib import Context
+    data, vel, prm, rows, refs = survey
+    print(f"production: migrating the whole volume, {storage} traces", flush=True)
+    ctx = Context(0)
+    try:
+        ctx.set_storage(storage)
+        out = P.migrate_volume(data, vel, prm, ctx=ctx)
+        img = out.cpu().numpy()
+        del out
+    finally:
+        ctx.close()
+    assert np.isfinite(img).all()
+    assert np.array_equal(img[:30], np.broadcast_to(img[30], img[:30].shape))
+    assert np.array_equal(img[431:], np.broadcast_to(img[430], img[431:].shape))
+    assert not img[:, :30].any() and not img[:, :, 0].any()
+    bar_l2, bar_trace = BARS[storage]
+    for xl0, m, ref in refs:
+        got = img[rows][:, xl0:xl0 + m]
+        err = O.rel_l2(got, ref)
+        worst = per_trace_rel(got, ref)
+        print(f"production {storage} window [{xl0}, {xl0 + m}): rel-L2 {err:.3e}, worst trace "
+              f"{worst:.3e}", flush=True)
+        assert err <= bar_l2, err
+        assert worst <= bar_trace, worst
