@@ -84,6 +84,7 @@ struct psm_ctx {
   std::mutex mu;
   std::string err;
   DevBuf raw, pairs, scale, vel, out, t0, tasks, rowpos, counters, dot, dotx, doty, tvpts, tvlab;
+  DevBuf adjrows, iorange;  // staged adjoint: row-pair windows of a pass, rows -> output inlines
   int storage = PSM_STORE_F32;
   // kernel-level timing of the last volume call (psm_enable_timing)
   bool timing = false;
@@ -92,6 +93,11 @@ struct psm_ctx {
   psm::Tuning tune;
   float alpha = 1.0f, alpha_mask = 1.0f;
 };
+
+// Device bytes the staged adjoint's row-pair buffer may take (passes of row
+// offsets keep it within this; the 201 x 201 x 751 least-squares survey needs
+// 4.9 GB for its 40 row offsets in one pass).
+constexpr size_t kAdjRowsBytes = (size_t)8 << 30;
 
 namespace {
 
@@ -302,7 +308,8 @@ void psm_destroy(psm_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->raw, &c->pairs, &c->scale, &c->vel, &c->out, &c->t0, &c->tasks,
-                    &c->rowpos, &c->counters, &c->dot, &c->dotx, &c->doty})
+                    &c->rowpos, &c->counters, &c->dot, &c->dotx, &c->doty, &c->tvpts, &c->tvlab,
+                    &c->adjrows, &c->iorange})
     b->release();
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -617,7 +624,55 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* in, const float* v
     if (p_g) *p_g = (double)h[1];
     return PSM_OK;
   }
-  PSM_HIP(c, psm::launch_migrate(a, mode, st, c->tune, extent));
+  // Staged adjoint: each column window is added once into a row-pair buffer
+  // [n_out_il][nd][n_xl][n_t] (row offsets d - d_lo of one pass) instead of
+  // into both data rows c + d and c - d, and a second kernel adds the buffer
+  // into the data rows (half the global atomics); passes of nd row offsets
+  // keep the buffer within kAdjRowsBytes.  rows_per_pass > 0 sets nd.
+  const size_t per_d = plan.n_out_il * (size_t)s->n_xl * nt;  // floats per row offset
+  int nd = 0;
+  if (adjoint && psm::adjoint_staged(a, c->tune) && per_d > 0) {
+    const size_t fit = kAdjRowsBytes / (per_d * sizeof(float));
+    nd = (int)std::min<size_t>(fit, (size_t)extent + 1);
+    if (c->tune.rows_per_pass > 0) nd = std::min(nd, c->tune.rows_per_pass);
+  }
+  if (nd > 0) {
+    // output inlines whose row window [b0, b1) holds data row r: contiguous,
+    // since c, b0 and b1 are nondecreasing over output inlines (else all)
+    std::vector<int> range(2 * s->n_il);
+    bool mono = true;
+    for (size_t io = 1; io < tasks.size(); ++io)
+      mono = mono && tasks[io].c >= tasks[io - 1].c && tasks[io].b0 >= tasks[io - 1].b0 &&
+             tasks[io].b1 >= tasks[io - 1].b1;
+    size_t lo = 0, hi = 0;
+    for (size_t r = 0; r < s->n_il; ++r) {
+      if (mono) {
+        while (lo < tasks.size() && tasks[lo].b1 <= (int)r) ++lo;
+        while (hi < tasks.size() && tasks[hi].b0 <= (int)r) ++hi;
+        range[2 * r] = (int)std::min(lo, hi);
+        range[2 * r + 1] = (int)hi;
+      } else {
+        range[2 * r] = 0;
+        range[2 * r + 1] = (int)tasks.size();
+      }
+    }
+    PSM_HIP(c, upload(c->iorange, range.data(), range.size() * sizeof(int), st));
+    PSM_ALLOC(c, c->adjrows, (size_t)nd * per_d * sizeof(float));
+    a.adj_rows = c->adjrows.as<float>();
+    a.adj_nd = nd;
+    for (int d0 = 0; d0 <= extent; d0 += nd) {
+      const int n_here = std::min(nd, extent + 1 - d0);
+      PSM_HIP(c, psm::launch_fill_zero(a.adj_rows, (size_t)nd * per_d, st));
+      a.d_lo = d0;
+      a.d_hi = d0 + n_here;
+      a.accumulate = d0 > 0 ? 1 : 0;
+      PSM_HIP(c, psm::launch_adjoint_staged(a, st));
+      PSM_HIP(c, psm::launch_adjoint_rows(a.adj_rows, d_out, w_tasks, c->iorange.as<int>(),
+                                          (int)s->n_il, (int)s->n_xl, (int)nt, nd, d0, st));
+    }
+  } else {
+    PSM_HIP(c, psm::launch_migrate(a, mode, st, c->tune, extent));
+  }
   if (timed) {
     PSM_HIP(c, hipEventRecord(c->ev[2], st));
     c->timed = true;

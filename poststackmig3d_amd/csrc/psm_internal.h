@@ -54,6 +54,11 @@ struct MigArgs {
   float alpha_mask;        // velocity scale of the aperture mask and window (1 = reference)
   const float* image;      // adjoint: image [n_out_il][n_out_xl][n_t] to scatter
   float* adj_out;          // adjoint: data [n_rows][n_xd][n_t], accumulated (zeroed by the caller)
+  float* adj_rows;         // staged adjoint: row-pair windows [n_out_il][adj_nd][n_xd][n_t]
+                           // at row offset d - d_lo, accumulated (zeroed by the caller) and
+                           // added into both rows of each pair afterwards; nullptr = the
+                           // windows are added into both data rows at once
+  int adj_nd;              // row offsets adj_rows holds per output inline
 };
 
 // The batch work queue is split by XCD (workgroups are dispatched to the 8
@@ -127,13 +132,26 @@ hipError_t launch_table_velocity(const psm_velocity_point* pts, int n_pts, float
                                  int n_t, double dt, float* vel, hipStream_t stream);
 
 hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream);
+// Staged adjoint: true if launch_migrate runs the adjoint in the staged form
+// for these arguments (contiguous rows, staging not switched off, rows within
+// kLdsMaxRowBytes).
+bool adjoint_staged(const MigArgs& a, const Tuning& tune);
+// One launch of the staged adjoint over the row offsets [a.d_lo, a.d_hi).
+hipError_t launch_adjoint_staged(const MigArgs& a, hipStream_t stream);
+// data[r][k][i] += sum of rows[io][|r - c| - d_lo][k][i] over the output
+// inlines io in [io_range[2 r], io_range[2 r + 1]) whose task (c, b0, b1)
+// holds row r (b0 <= r < b1) at a row offset |r - c| in [d_lo, d_lo + nd):
+// the row-pair windows of one staged-adjoint pass added into both rows of
+// each pair.  tasks and io_range are device memory.
+hipError_t launch_adjoint_rows(const float* rows, float* data, const Task* tasks,
+                               const int* io_range, int n_rows, int n_xd, int n_t, int nd,
+                               int d_lo, hipStream_t stream);
 // Persistent batch launch (forward migration of several volumes): `blocks`
 // workgroups of wpb waves drain the kBatchXcds queues described by segs;
 // counters[x] (zero at launch) counts the positions taken from XCD x's queue,
 // totals[x] its length (both device memory).  staged: workgroups of the
 // staged form with wpb (4 or 8) output crosslines and stage_g (2 or 4)
-// offsets per group (FP16/BF16: 8 and 4 only), else the gather form with wpb
-// waves per workgroup.
+// offsets per group, else the gather form with wpb waves per workgroup.
 hipError_t launch_batch(int storage, bool staged, int stage_g, const MigArgs* items,
                         const BatchSeg* segs,
                         int n_segs, unsigned* counters, const unsigned* totals, int wpb,

@@ -1074,7 +1074,16 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
               win[lane] = F2{0.0f, 0.0f};
               win[lane + kWave] = F2{0.0f, 0.0f};
             }
-            if (k >= xd_lo && k < xd_hi) {  // outside the data: nothing scattered there counts
+            if (k >= xd_lo && k < xd_hi && a.adj_rows) {
+              // the window once, into the pass's row-pair buffer (added into
+              // both rows of the pair by launch_adjoint_rows)
+              float* const dst =
+                  a.adj_rows +
+                  (((size_t)io * a.adj_nd + (d - a.d_lo)) * a.n_xd + (k - xd_lo)) * a.n_t + lo +
+                  lane;
+              if (lane < n && v0 != 0.0f) unsafeAtomicAdd(dst, v0);
+              if (lane + kWave < n && v1 != 0.0f) unsafeAtomicAdd(dst + kWave, v1);
+            } else if (k >= xd_lo && k < xd_hi) {  // outside the data: nothing scattered there counts
               float* const dst0 = a.adj_out + ((size_t)r0 * a.n_xd + (k - xd_lo)) * a.n_t + lo + lane;
               if (lane < n && v0 != 0.0f) unsafeAtomicAdd(dst0, v0);
               if (lane + kWave < n && v1 != 0.0f) unsafeAtomicAdd(dst0 + kWave, v1);
@@ -1725,6 +1734,36 @@ __global__ void fill_zero_kernel(float* __restrict__ p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
 }
 
+// Row-pair windows of one staged-adjoint pass into the data cube: the block
+// (r, k, 256-sample chunk) sums, in output-inline order, the window of every
+// output inline io whose task holds row r at a row offset of the pass, and
+// adds the sum into data[r][k][chunk] (each sample has one thread: no
+// atomics).  The task and range loads are uniform (scalar).
+__global__ __launch_bounds__(256) void adjoint_rows_kernel(
+    const float* __restrict__ rows, float* __restrict__ data, const Task* __restrict__ tasks,
+    const int* __restrict__ io_range, int n_rows, int n_xd, int n_t, int nd, int d_lo,
+    int n_chunks) {
+  const int blk = blockIdx.x;
+  const int chunk = blk % n_chunks;
+  const int rk = blk / n_chunks;
+  const int r = rk / n_xd, k = rk % n_xd;
+  const int i = chunk * 256 + (int)threadIdx.x;
+  if (r >= n_rows) return;
+  const int io_lo = io_range[2 * r], io_hi = io_range[2 * r + 1];
+  float acc = 0.0f;
+  if (i < n_t) {
+#pragma unroll 4
+    for (int io = io_lo; io < io_hi; ++io) {
+      const Task tk = tasks[io];
+      const int d = r >= tk.c ? r - tk.c : tk.c - r;
+      if (d < d_lo || d >= d_lo + nd || r < tk.b0 || r >= tk.b1) continue;
+      acc += rows[(((size_t)io * nd + (d - d_lo)) * n_xd + k) * n_t + i];
+    }
+    float* const o = data + ((size_t)r * n_xd + k) * n_t + i;
+    *o += acc;
+  }
+}
+
 // ---------------------------------------------------------------- packing
 // Pair layout [n_rows + 1][n_xd + 1][n_t]: sample n_t-1 of every trace, the
 // last trace of every row and the whole last row are zero.  F32 stores
@@ -1905,6 +1944,28 @@ hipError_t launch_adj_lds(const MigArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+bool adjoint_staged(const MigArgs& a, const Tuning& tune) {
+  return a.row_pos == nullptr && tune.staging != 0 &&
+         (size_t)(a.n_xd + 1) * a.n_t * 4 <= kLdsMaxRowBytes;
+}
+
+hipError_t launch_adjoint_staged(const MigArgs& a, hipStream_t stream) {
+  if (a.n_out_il <= 0 || a.n_out_xl <= 0 || a.n_t <= 0) return hipSuccess;
+  return launch_adj_lds(a, stream);
+}
+
+hipError_t launch_adjoint_rows(const float* rows, float* data, const Task* tasks,
+                               const int* io_range, int n_rows, int n_xd, int n_t, int nd,
+                               int d_lo, hipStream_t stream) {
+  if (n_rows <= 0 || n_xd <= 0 || n_t <= 0 || nd <= 0) return hipSuccess;
+  const int n_chunks = (n_t + 255) / 256;
+  const long long blocks = (long long)n_rows * n_xd * n_chunks;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(adjoint_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, rows,
+                     data, tasks, io_range, n_rows, n_xd, n_t, nd, d_lo, n_chunks);
+  return hipGetLastError();
+}
+
 template <int ST>
 hipError_t launch_lds_st(const MigArgs& a, hipStream_t stream, int wpb, int steps) {
   if (wpb == 8)
@@ -1939,8 +2000,7 @@ hipError_t launch_migrate(const MigArgs& a0, int mode, hipStream_t stream, const
   const int steps = auto_lds ? kAutoStageSteps : tune.stage_steps;
   const bool lds = auto_lds || (tune.staging > 0 && lds_ok && (wpb == 4 || wpb == 8));
   // staged adjoint: contiguous rows, unless staging is switched off
-  const bool adj_lds = mode == kAdjoint && a0.row_pos == nullptr && tune.staging != 0 &&
-                       (size_t)(a0.n_xd + 1) * a0.n_t * 4 <= kLdsMaxRowBytes;
+  const bool adj_lds = mode == kAdjoint && adjoint_staged(a0, tune);
   // automatic passes: 8 row offsets per launch for FP32 gathers, one launch otherwise
   const int rpp = tune.rows_per_pass >= 0 ? tune.rows_per_pass
                                           : ((f32_traces && !lds && !adj_lds) ? kAutoRowsF32 : 0);

@@ -138,11 +138,14 @@ def test_staged_matches_reference_and_gather(ctx, name):
 @pytest.mark.parametrize("name", CASE_NAMES)
 def test_staged_adjoint_matches_gather_adjoint(ctx, name):
     """The staged adjoint (the workgroup's image samples scattered into LDS
-    windows, each window flushed once into the data cube) against the
-    gather-form adjoint (every pair scattered with global atomics): the same
-    terms summed in another order.  The rows whose windows or offsets do not
-    fit (coarse_slow, fine_bins) and sticky workgroups take the gather-form
-    adjoint inside the staged kernel."""
+    windows, each window flushed once into a row-pair buffer that a second
+    kernel adds into both data rows of the pair) against the gather-form
+    adjoint (every pair scattered with global atomics): the same terms summed
+    in another order.  The rows whose windows or offsets do not fit
+    (coarse_slow, fine_bins) and sticky workgroups take the gather-form
+    adjoint inside the staged kernel.  Passes of 1 and 3 row offsets (the
+    row-pair buffer holding part of the aperture at a time) give the same
+    data to rounding."""
     data, vel, prm, _ = _case(name)
     info = P.survey_plan(prm, vel)
     rng = np.random.default_rng(11)
@@ -158,6 +161,15 @@ def test_staged_adjoint_matches_gather_adjoint(ctx, name):
     e = rel_l2(staged, gather)
     print(f"  staged vs gather adjoint {e:.2e}", flush=True)
     assert e <= TOL_FORMS, e
+    for rows in (1, 3):
+        ctx.set_tuning(4, rows)
+        try:
+            passes = P.demigrate_volume(image, vel, prm, ctx=ctx)
+        finally:
+            ctx.set_tuning(4, -1)
+        e = rel_l2(passes, gather)
+        print(f"  staged adjoint in passes of {rows} row offsets vs gather {e:.2e}", flush=True)
+        assert e <= TOL_FORMS, (rows, e)
 
 
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
