@@ -84,7 +84,7 @@ struct psm_ctx {
   std::mutex mu;
   std::string err;
   DevBuf raw, pairs, scale, vel, out, t0, tasks, rowpos, counters, dot, dotx, doty, tvpts, tvlab;
-  DevBuf adjrows, iorange;  // staged adjoint: row-pair windows of a pass, rows -> output inlines
+  DevBuf adjrows, iorange, ioplane;  // staged adjoint: row-pair windows of a pass, their lists per data row
   int storage = PSM_STORE_F32;
   // kernel-level timing of the last volume call (psm_enable_timing)
   bool timing = false;
@@ -309,7 +309,7 @@ void psm_destroy(psm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->raw, &c->pairs, &c->scale, &c->vel, &c->out, &c->t0, &c->tasks,
                     &c->rowpos, &c->counters, &c->dot, &c->dotx, &c->doty, &c->tvpts, &c->tvlab,
-                    &c->adjrows, &c->iorange})
+                    &c->adjrows, &c->iorange, &c->ioplane})
     b->release();
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -637,38 +637,43 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* in, const float* v
     if (c->tune.rows_per_pass > 0) nd = std::min(nd, c->tune.rows_per_pass);
   }
   if (nd > 0) {
-    // output inlines whose row window [b0, b1) holds data row r: contiguous,
-    // since c, b0 and b1 are nondecreasing over output inlines (else all)
-    std::vector<int> range(2 * s->n_il);
-    bool mono = true;
-    for (size_t io = 1; io < tasks.size(); ++io)
-      mono = mono && tasks[io].c >= tasks[io - 1].c && tasks[io].b0 >= tasks[io - 1].b0 &&
-             tasks[io].b1 >= tasks[io - 1].b1;
-    size_t lo = 0, hi = 0;
-    for (size_t r = 0; r < s->n_il; ++r) {
-      if (mono) {
-        while (lo < tasks.size() && tasks[lo].b1 <= (int)r) ++lo;
-        while (hi < tasks.size() && tasks[hi].b0 <= (int)r) ++hi;
-        range[2 * r] = (int)std::min(lo, hi);
-        range[2 * r + 1] = (int)hi;
-      } else {
-        range[2 * r] = 0;
-        range[2 * r + 1] = (int)tasks.size();
-      }
-    }
-    PSM_HIP(c, upload(c->iorange, range.data(), range.size() * sizeof(int), st));
     PSM_ALLOC(c, c->adjrows, (size_t)nd * per_d * sizeof(float));
     a.adj_rows = c->adjrows.as<float>();
     a.adj_nd = nd;
-    for (int d0 = 0; d0 <= extent; d0 += nd) {
-      const int n_here = std::min(nd, extent + 1 - d0);
+    // per pass and data row r, the planes io * nd + (d - d0) of the
+    // row-pair buffer whose pair holds r: output inlines io with
+    // b0 <= r < b1 and d = |r - c(io)| in the pass, in output-inline order;
+    // all passes' lists are uploaded once, before the first launch
+    const size_t n_rows = s->n_il;
+    const int n_pass = extent / nd + 1;
+    std::vector<int> start((size_t)n_pass * (n_rows + 1)), plane;
+    for (int p = 0; p < n_pass; ++p) {
+      const int d0 = p * nd, d1 = std::min(d0 + nd, extent + 1);
+      int* const st_p = start.data() + (size_t)p * (n_rows + 1);
+      for (size_t r = 0; r < n_rows; ++r) {
+        st_p[r] = (int)plane.size();
+        for (size_t io = 0; io < tasks.size(); ++io) {
+          const psm::Task& tk = tasks[io];
+          const int d = std::abs((int)r - tk.c);
+          if ((int)r >= tk.b0 && (int)r < tk.b1 && d >= d0 && d < d1)
+            plane.push_back((int)io * nd + (d - d0));
+        }
+      }
+      st_p[n_rows] = (int)plane.size();
+    }
+    PSM_HIP(c, upload(c->iorange, start.data(), start.size() * sizeof(int), st));
+    PSM_HIP(c, upload(c->ioplane, plane.data(), std::max<size_t>(plane.size(), 1) * sizeof(int), st));
+    for (int p = 0; p < n_pass; ++p) {
+      const int d0 = p * nd;
       PSM_HIP(c, psm::launch_fill_zero(a.adj_rows, (size_t)nd * per_d, st));
       a.d_lo = d0;
-      a.d_hi = d0 + n_here;
+      a.d_hi = std::min(d0 + nd, extent + 1);
       a.accumulate = d0 > 0 ? 1 : 0;
       PSM_HIP(c, psm::launch_adjoint_staged(a, st));
-      PSM_HIP(c, psm::launch_adjoint_rows(a.adj_rows, d_out, w_tasks, c->iorange.as<int>(),
-                                          (int)s->n_il, (int)s->n_xl, (int)nt, nd, d0, st));
+      PSM_HIP(c, psm::launch_adjoint_rows(a.adj_rows, d_out,
+                                          c->iorange.as<int>() + (size_t)p * (n_rows + 1),
+                                          c->ioplane.as<int>(), (int)n_rows, (int)s->n_xl,
+                                          (int)nt, st));
     }
   } else {
     PSM_HIP(c, psm::launch_migrate(a, mode, st, c->tune, extent));

@@ -138,14 +138,13 @@ hipError_t launch_fill_zero(float* p, size_t n, hipStream_t stream);
 bool adjoint_staged(const MigArgs& a, const Tuning& tune);
 // One launch of the staged adjoint over the row offsets [a.d_lo, a.d_hi).
 hipError_t launch_adjoint_staged(const MigArgs& a, hipStream_t stream);
-// data[r][k][i] += sum of rows[io][|r - c| - d_lo][k][i] over the output
-// inlines io in [io_range[2 r], io_range[2 r + 1]) whose task (c, b0, b1)
-// holds row r (b0 <= r < b1) at a row offset |r - c| in [d_lo, d_lo + nd):
-// the row-pair windows of one staged-adjoint pass added into both rows of
-// each pair.  tasks and io_range are device memory.
-hipError_t launch_adjoint_rows(const float* rows, float* data, const Task* tasks,
-                               const int* io_range, int n_rows, int n_xd, int n_t, int nd,
-                               int d_lo, hipStream_t stream);
+// data[r][k][i] += sum over e in [start[r], start[r + 1]) of
+// rows[plane[e]][k][i] (rows: planes of n_xd x n_t floats): the row-pair
+// windows of one staged-adjoint pass added into both rows of each pair, the
+// planes of data row r listed in output-inline order.  start and plane are
+// device memory.
+hipError_t launch_adjoint_rows(const float* rows, float* data, const int* start, const int* plane,
+                               int n_rows, int n_xd, int n_t, hipStream_t stream);
 // Persistent batch launch (forward migration of several volumes): `blocks`
 // workgroups of wpb waves drain the kBatchXcds queues described by segs;
 // counters[x] (zero at launch) counts the positions taken from XCD x's queue,

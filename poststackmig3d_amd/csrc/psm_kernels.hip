@@ -1735,33 +1735,28 @@ __global__ void fill_zero_kernel(float* __restrict__ p, size_t n) {
 }
 
 // Row-pair windows of one staged-adjoint pass into the data cube: the block
-// (r, k, 256-sample chunk) sums, in output-inline order, the window of every
-// output inline io whose task holds row r at a row offset of the pass, and
-// adds the sum into data[r][k][chunk] (each sample has one thread: no
-// atomics).  The task and range loads are uniform (scalar).
+// (r, k, 256-sample chunk) sums the planes plane[e] of the row-pair buffer
+// listed for data row r (entries start[r] .. start[r + 1] - 1, each an output
+// inline and row offset whose pair holds row r, in output-inline order) and
+// adds the sum into data[r][k][chunk].  Each sample has one thread (no
+// atomics); the list is uniform and the loads independent.
 __global__ __launch_bounds__(256) void adjoint_rows_kernel(
-    const float* __restrict__ rows, float* __restrict__ data, const Task* __restrict__ tasks,
-    const int* __restrict__ io_range, int n_rows, int n_xd, int n_t, int nd, int d_lo,
-    int n_chunks) {
+    const float* __restrict__ rows, float* __restrict__ data, const int* __restrict__ start,
+    const int* __restrict__ plane, int n_rows, int n_xd, int n_t, int n_chunks) {
   const int blk = blockIdx.x;
   const int chunk = blk % n_chunks;
   const int rk = blk / n_chunks;
   const int r = rk / n_xd, k = rk % n_xd;
   const int i = chunk * 256 + (int)threadIdx.x;
-  if (r >= n_rows) return;
-  const int io_lo = io_range[2 * r], io_hi = io_range[2 * r + 1];
+  if (r >= n_rows || i >= n_t) return;
+  const int e0 = start[r], e1 = start[r + 1];
+  const size_t pstride = (size_t)n_xd * n_t;
+  const float* const src = rows + (size_t)k * n_t + i;
   float acc = 0.0f;
-  if (i < n_t) {
-#pragma unroll 4
-    for (int io = io_lo; io < io_hi; ++io) {
-      const Task tk = tasks[io];
-      const int d = r >= tk.c ? r - tk.c : tk.c - r;
-      if (d < d_lo || d >= d_lo + nd || r < tk.b0 || r >= tk.b1) continue;
-      acc += rows[(((size_t)io * nd + (d - d_lo)) * n_xd + k) * n_t + i];
-    }
-    float* const o = data + ((size_t)r * n_xd + k) * n_t + i;
-    *o += acc;
-  }
+#pragma unroll 8
+  for (int e = e0; e < e1; ++e) acc += src[(size_t)plane[e] * pstride];
+  float* const o = data + ((size_t)r * n_xd + k) * n_t + i;
+  *o += acc;
 }
 
 // ---------------------------------------------------------------- packing
@@ -1954,15 +1949,14 @@ hipError_t launch_adjoint_staged(const MigArgs& a, hipStream_t stream) {
   return launch_adj_lds(a, stream);
 }
 
-hipError_t launch_adjoint_rows(const float* rows, float* data, const Task* tasks,
-                               const int* io_range, int n_rows, int n_xd, int n_t, int nd,
-                               int d_lo, hipStream_t stream) {
-  if (n_rows <= 0 || n_xd <= 0 || n_t <= 0 || nd <= 0) return hipSuccess;
+hipError_t launch_adjoint_rows(const float* rows, float* data, const int* start, const int* plane,
+                               int n_rows, int n_xd, int n_t, hipStream_t stream) {
+  if (n_rows <= 0 || n_xd <= 0 || n_t <= 0) return hipSuccess;
   const int n_chunks = (n_t + 255) / 256;
   const long long blocks = (long long)n_rows * n_xd * n_chunks;
   if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
   hipLaunchKernelGGL(adjoint_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, rows,
-                     data, tasks, io_range, n_rows, n_xd, n_t, nd, d_lo, n_chunks);
+                     data, start, plane, n_rows, n_xd, n_t, n_chunks);
   return hipGetLastError();
 }
 
