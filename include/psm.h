@@ -178,6 +178,9 @@ int psm_migrate_volume_noalloc(psm_ctx* ctx, const psm_survey* survey,
  * d[i] += w (1 - f) m, d[i+1] += w f m, so <M x, y> = <x, M^T y> up to float
  * rounding.  Crossline padding traces are zero inputs of the forward
  * operator and receive nothing.  Memory per `where`, as psm_migrate_volume.
+ * The staged form keeps a row-pair buffer of up to 8 GiB on the context
+ * (one plane per output inline and row offset of a pass; psm_set_tuning's
+ * rows_per_pass > 0 caps the row offsets per pass).
  */
 int psm_demigrate_volume(psm_ctx* ctx, const psm_survey* survey,
                          const float* image, const float* vel, float* data,
@@ -261,8 +264,9 @@ int psm_count_pairs(psm_ctx* ctx, const psm_survey* survey, const float* vel,
  * aperture in one launch; smaller values bound each launch's cache working
  * set to a few aperture rows, later launches accumulate; -1, the default =
  * automatic: one launch for the staged form and for FP16/BF16 gathers, 8
- * for FP32 gathers).  Results
- * do not depend on the shape beyond FP32 summation order. */
+ * for FP32 gathers; the staged adjoint: as many row offsets per pass as its
+ * 8 GiB row-pair buffer holds).  Results do not depend on the shape beyond
+ * FP32 summation order. */
 int psm_set_tuning(psm_ctx* ctx, int waves_per_block, int rows_per_pass);
 int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
 
@@ -279,8 +283,10 @@ int psm_get_tuning(psm_ctx* ctx, int* waves_per_block, int* rows_per_pass);
  * The staged form
  * applies to whole-volume calls and per-inline calls with contiguous rows;
  * other calls use the gather form.  The switch also selects the adjoint's
- * form (psm_demigrate_volume): staged (LDS accumulator windows, one per
- * column and wave, 8 output crosslines per workgroup) unless on = 0.
+ * form (psm_demigrate_volume): staged (LDS accumulator windows per column,
+ * 8 output crosslines per workgroup, each window added once into a
+ * row-pair buffer that a second kernel adds into both data rows of the
+ * pair) unless on = 0.
  * Results do not depend on the form beyond FP32 summation order. */
 int psm_set_staging(psm_ctx* ctx, int on, int steps);
 int psm_get_staging(psm_ctx* ctx, int* on, int* steps);
