@@ -924,10 +924,10 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         if (!quiet) __syncthreads();  // the previous row's windows and tables are no longer read
         quiet = false;
         // ---- column tables: meta[side][o] = LDS address of stored pair 0 of
-        // the column's window (slot - lo * kB); wave w fills columns
-        // 64 w .. 64 w + 63 (and 64 (w + W) ... when a row has more than
-        // 64 W), so only the first nO / 64 waves spend instructions on it and
-        // the table writes are conflict-free.
+        // the column's window (slot - lo * kB); each wave fills one side of
+        // 64 consecutive columns at a time (the table writes are
+        // conflict-free), so only the first nO / 32 waves spend instructions
+        // on it and the pass ends after one side's work per wave.
         // Column o of side s is read at the offsets dk in [max(s, o-W+1),
         // min(o, kmax)] (one per wave), so q of every read lies between
         // sqrt(t0lo^2 + alo r^2(dk_lo)) / dt and sqrt(t0hi^2 + ahi r^2(dk_hi)) / dt
@@ -938,56 +938,54 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // issued for columns past the row store into the spare slot); their
         // widths are zeroed, so those loads read nothing.
         constexpr int kPast = 4 * G + W;
-        for (int ob = w * kWave; ob < nO + kPast; ob += W * kWave) {
-          const int o = ob + lane;
+        // table job tj of wave w (tj = w, w + W, ...): side tj % 2 of columns
+        // 64 (tj / 2) .. 64 (tj / 2) + 63, so a row of up to 4 W - 24
+        // columns takes one pass of half a wave's column work per wave
+        for (int tj = w; (tj >> 1) * kWave < nO + kPast; tj += W) {
+          const int side = tj & 1;
+          const int o = (tj >> 1) * kWave + lane;
+          int* const wid = reinterpret_cast<int*>(smem + PL::kWid) + side * PL::kCols + o;
           if (o >= nO) {
-            if (o < nO + kPast) {
-              reinterpret_cast<int*>(smem + PL::kWid)[o] = 0;
-              reinterpret_cast<int*>(smem + PL::kWid)[PL::kCols + o] = 0;
-            }
+            if (o < nO + kPast) *wid = 0;
           } else {
-#pragma unroll
-            for (int side = 0; side < 2; ++side) {
-              const int dlo = max(side, o - (W - 1)), dhi = min(o, kmax_wg);
-              int lo = 0;
-              if (dlo <= dhi) {
-                const float ylo = (float)dlo * a.dy, yhi = (float)dhi * a.dy;
-                const float lo_t = __builtin_amdgcn_sqrtf(
-                    __builtin_fmaf(gAlo, __builtin_fmaf(ylo, ylo, d2), gTlo2));
-                const float hi_t = __builtin_amdgcn_sqrtf(
-                    __builtin_fmaf(gAhi, __builtin_fmaf(yhi, yhi, d2), gThi2));
-                // one index of slack each way covers the rounding of q and
-                // the one-ulp error of v_sqrt_f32 (a relative 1.2e-7 of q)
-                int* const wid = reinterpret_cast<int*>(smem + PL::kWid) + side * PL::kCols + o;
-                if constexpr (PL::kOpenEnd) {
-                  // The window is not cut at the trace end; only pairs up to
-                  // n_t - 1 are loaded and the lanes past them load zeros.
-                  // The lane holding pair n_t - 1 must not also hold pair n_t
-                  // (the next trace's first), so where the window reaches
-                  // past the end lo is lowered by one to make n_t - 1 - lo
-                  // odd.
-                  lo = max((int)(lo_t * a.inv_dt) - 1, 0);
-                  const int hi = max((int)(hi_t * a.inv_dt) + 1, 0);
-                  if (hi > nt1 && ((nt1 - lo) & 1) == 0) {
-                    if (lo > 0)
-                      --lo;
-                    else
-                      ovf = true;
-                  }
-                  ovf = ovf || (hi - lo >= kWinPairs);
-                  *wid = lo <= nt1 ? (min(hi, nt1) - lo + 1) * kB : 0;
-                } else {
-                  lo = min(max((int)(lo_t * a.inv_dt) - 1, 0), nt1);
-                  const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
-                  // (the adjoint also touches sample hi + 1)
-                  ovf = ovf || (hi - lo >= (PL::kAdj ? kWinPairs - 1 : kWinPairs));
-                  *wid = PL::kAdj ? min(hi + 1, nt1) - lo + 1 : (hi - lo + 1) * kB;
+            const int dlo = max(side, o - (W - 1)), dhi = min(o, kmax_wg);
+            int lo = 0;
+            if (dlo <= dhi) {
+              const float ylo = (float)dlo * a.dy, yhi = (float)dhi * a.dy;
+              const float lo_t = __builtin_amdgcn_sqrtf(
+                  __builtin_fmaf(gAlo, __builtin_fmaf(ylo, ylo, d2), gTlo2));
+              const float hi_t = __builtin_amdgcn_sqrtf(
+                  __builtin_fmaf(gAhi, __builtin_fmaf(yhi, yhi, d2), gThi2));
+              // one index of slack each way covers the rounding of q and
+              // the one-ulp error of v_sqrt_f32 (a relative 1.2e-7 of q)
+              if constexpr (PL::kOpenEnd) {
+                // The window is not cut at the trace end; only pairs up to
+                // n_t - 1 are loaded and the lanes past them load zeros.
+                // The lane holding pair n_t - 1 must not also hold pair n_t
+                // (the next trace's first), so where the window reaches
+                // past the end lo is lowered by one to make n_t - 1 - lo
+                // odd.
+                lo = max((int)(lo_t * a.inv_dt) - 1, 0);
+                const int hi = max((int)(hi_t * a.inv_dt) + 1, 0);
+                if (hi > nt1 && ((nt1 - lo) & 1) == 0) {
+                  if (lo > 0)
+                    --lo;
+                  else
+                    ovf = true;
                 }
+                ovf = ovf || (hi - lo >= kWinPairs);
+                *wid = lo <= nt1 ? (min(hi, nt1) - lo + 1) * kB : 0;
               } else {
-                reinterpret_cast<int*>(smem + PL::kWid)[side * PL::kCols + o] = 0;
+                lo = min(max((int)(lo_t * a.inv_dt) - 1, 0), nt1);
+                const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
+                // (the adjoint also touches sample hi + 1)
+                ovf = ovf || (hi - lo >= (PL::kAdj ? kWinPairs - 1 : kWinPairs));
+                *wid = PL::kAdj ? min(hi + 1, nt1) - lo + 1 : (hi - lo + 1) * kB;
               }
-              meta[side * PL::kCols + o] = PL::slot(side, o) - lo * kB;
+            } else {
+              *wid = 0;
             }
+            meta[side * PL::kCols + o] = PL::slot(side, o) - lo * kB;
           }
         }
         for (int dk = w * kWave + lane; dk <= kmax_wg; dk += W * kWave) {
