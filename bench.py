@@ -95,7 +95,7 @@ POINT = dict(shape=(101, 101, 501),
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["production", "point", "scan", "lsm", "e2e", "dropin"],
                     default="production")
@@ -834,6 +834,8 @@ def main():
                    "parallelism": "single GPU"},
         "value_p_g": p_g / (wall / args.steps),
         "kernel_ms": round(avg_kernel_s * 1e3, 3),
+        "kernel_ms_median": round(statistics.median(kernel_ms), 3),
+        "step_ms_median": round(statistics.median(step_ms), 3),
         "pack_ms": round(sum(pack_ms) / len(pack_ms), 3),
         "roofline": roofline,
         "parity": parity,
