@@ -877,6 +877,60 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
     // row resumes from the previous row's answer and steps down.  km_run:
     // lane l < W, wave l's sweep (clipped to offsets that can hold data);
     // kin_run: every valid lane of this wave inside the aperture.
+    // ---- staging helpers (forward)
+    using V4 = unsigned __attribute__((ext_vector_type(4)));
+    using V2 = unsigned __attribute__((ext_vector_type(2)));
+    using F4 = float __attribute__((ext_vector_type(4)));
+    using JV = std::conditional_t<S::kScaled, V2, V4>;  // one row of a job, per lane
+    const int lb = lane * 16;
+    auto load_row = [&](__amdgpu_buffer_rsrc_t r, int voff, int soff) -> JV {
+      if constexpr (S::kScaled)
+        return __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+      else
+        return __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    };
+    // the two stored pairs of one row as FP32 pairs (sample, difference),
+    // unscaled
+    auto unpack = [&](JV v) -> F4 {
+      if constexpr (ST == PSM_STORE_F32) {
+        return __builtin_bit_cast(F4, v);
+      } else if constexpr (ST == PSM_STORE_F16) {
+        return F4{(float)__builtin_bit_cast(_Float16, (unsigned short)(v.x & 0xffffu)),
+                  (float)__builtin_bit_cast(_Float16, (unsigned short)(v.x >> 16)),
+                  (float)__builtin_bit_cast(_Float16, (unsigned short)(v.y & 0xffffu)),
+                  (float)__builtin_bit_cast(_Float16, (unsigned short)(v.y >> 16))};
+      } else {
+        return F4{__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xffff0000u),
+                  __uint_as_float(v.y << 16), __uint_as_float(v.y & 0xffff0000u)};
+      }
+    };
+    // one row's pairs times its trace's scale s (F16 / BF16; FP32 rows
+    // are not scaled).  fma(x, s, 0) is x s but for the sign of an exact
+    // zero (no matter in a sum), and lets an F16 x be widened and scaled
+    // by one v_fma_mix_f32.
+    auto widen = [&](JV v, float s) -> F4 {
+      if constexpr (S::kScaled) {
+        const F4 x = unpack(v);
+        return F4{__builtin_fmaf(x.x, s, 0.0f), __builtin_fmaf(x.y, s, 0.0f),
+                  __builtin_fmaf(x.z, s, 0.0f), __builtin_fmaf(x.w, s, 0.0f)};
+      }
+      return unpack(v);
+    };
+    // acc + the row's pairs (times s for F16 / BF16)
+    auto widen_add = [&](JV v, float s, F4 acc) -> F4 {
+      const F4 x = unpack(v);
+      if constexpr (S::kScaled)
+        return F4{__builtin_fmaf(x.x, s, acc.x), __builtin_fmaf(x.y, s, acc.y),
+                  __builtin_fmaf(x.z, s, acc.z), __builtin_fmaf(x.w, s, acc.w)};
+      return acc + x;
+    };
+    // A job's source: column o, side of the job index j within the
+    // columns [oa, oa + nc).
+    // (one return statement: a lambda returning this struct on two paths
+    // made the compiler copy the kernel arguments to scratch)
+    auto job_at = [&](int oa, int nc, int j) {
+      return StageJob{oa + (j >> 1), j & 1, 0, j < 2 * nc};
+    };
     int km_run = -1;
     if (pl && pR2 >= 0.0f) {
       const int kjl = kbase + lane;
@@ -919,6 +973,47 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
       const int r0 = hp ? rp : rm;  // ring row 0; ring row 1 = rm when both rows exist
       const bool two = hp && hm;
 
+      // Window of column o of `side` in this row: stored pairs lo .. hi any
+      // wave's read can touch (wbytes = LDS bytes of pairs lo .. min(hi,
+      // n_t - 1); 0 if no wave reads the column); ovf |= the window exceeds
+      // a slot.
+      auto col_window = [&](int side, int o, int& lo, int& wbytes, bool& ovf) {
+        const int dlo = max(side, o - (W - 1)), dhi = min(o, kmax_wg);
+        lo = 0;
+        wbytes = 0;
+        if (dlo <= dhi) {
+          const float ylo = (float)dlo * a.dy, yhi = (float)dhi * a.dy;
+          const float lo_t = __builtin_amdgcn_sqrtf(
+              __builtin_fmaf(gAlo, __builtin_fmaf(ylo, ylo, d2), gTlo2));
+          const float hi_t = __builtin_amdgcn_sqrtf(
+              __builtin_fmaf(gAhi, __builtin_fmaf(yhi, yhi, d2), gThi2));
+          // one index of slack each way covers the rounding of q and the
+          // one-ulp error of v_sqrt_f32 (a relative 1.2e-7 of q)
+          if constexpr (PL::kOpenEnd) {
+            // The window is not cut at the trace end; only pairs up to
+            // n_t - 1 are loaded and the lanes past them load zeros.  The
+            // lane holding pair n_t - 1 must not also hold pair n_t (the
+            // next trace's first), so where the window reaches past the end
+            // lo is lowered by one to make n_t - 1 - lo odd.
+            lo = max((int)(lo_t * a.inv_dt) - 1, 0);
+            const int hi = max((int)(hi_t * a.inv_dt) + 1, 0);
+            if (hi > nt1 && ((nt1 - lo) & 1) == 0) {
+              if (lo > 0)
+                --lo;
+              else
+                ovf = true;
+            }
+            ovf = ovf || (hi - lo >= kWinPairs);
+            wbytes = lo <= nt1 ? (min(hi, nt1) - lo + 1) * kB : 0;
+          } else {
+            lo = min(max((int)(lo_t * a.inv_dt) - 1, 0), nt1);
+            const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
+            // (the adjoint also touches sample hi + 1)
+            ovf = ovf || (hi - lo >= (PL::kAdj ? kWinPairs - 1 : kWinPairs));
+            wbytes = PL::kAdj ? min(hi + 1, nt1) - lo + 1 : (hi - lo + 1) * kB;
+          }
+        }
+      };
       bool fallback = nO > PL::kMaxColsP;
       if (!fallback) {
         if (!quiet) __syncthreads();  // the previous row's windows and tables are no longer read
@@ -948,43 +1043,9 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           if (o >= nO) {
             if (o < nO + kPast) *wid = 0;
           } else {
-            const int dlo = max(side, o - (W - 1)), dhi = min(o, kmax_wg);
-            int lo = 0;
-            if (dlo <= dhi) {
-              const float ylo = (float)dlo * a.dy, yhi = (float)dhi * a.dy;
-              const float lo_t = __builtin_amdgcn_sqrtf(
-                  __builtin_fmaf(gAlo, __builtin_fmaf(ylo, ylo, d2), gTlo2));
-              const float hi_t = __builtin_amdgcn_sqrtf(
-                  __builtin_fmaf(gAhi, __builtin_fmaf(yhi, yhi, d2), gThi2));
-              // one index of slack each way covers the rounding of q and
-              // the one-ulp error of v_sqrt_f32 (a relative 1.2e-7 of q)
-              if constexpr (PL::kOpenEnd) {
-                // The window is not cut at the trace end; only pairs up to
-                // n_t - 1 are loaded and the lanes past them load zeros.
-                // The lane holding pair n_t - 1 must not also hold pair n_t
-                // (the next trace's first), so where the window reaches
-                // past the end lo is lowered by one to make n_t - 1 - lo
-                // odd.
-                lo = max((int)(lo_t * a.inv_dt) - 1, 0);
-                const int hi = max((int)(hi_t * a.inv_dt) + 1, 0);
-                if (hi > nt1 && ((nt1 - lo) & 1) == 0) {
-                  if (lo > 0)
-                    --lo;
-                  else
-                    ovf = true;
-                }
-                ovf = ovf || (hi - lo >= kWinPairs);
-                *wid = lo <= nt1 ? (min(hi, nt1) - lo + 1) * kB : 0;
-              } else {
-                lo = min(max((int)(lo_t * a.inv_dt) - 1, 0), nt1);
-                const int hi = min(max((int)(hi_t * a.inv_dt) + 1, 0), nt1);
-                // (the adjoint also touches sample hi + 1)
-                ovf = ovf || (hi - lo >= (PL::kAdj ? kWinPairs - 1 : kWinPairs));
-                *wid = PL::kAdj ? min(hi + 1, nt1) - lo + 1 : (hi - lo + 1) * kB;
-              }
-            } else {
-              *wid = 0;
-            }
+            int lo, wb;
+            col_window(side, o, lo, wb, ovf);
+            *wid = wb;
             meta[side * PL::kCols + o] = PL::slot(side, o) - lo * kB;
           }
         }
@@ -996,7 +1057,51 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // l, ...)
         const bool wave_ovf = __any(ovf);
         if (lane == 0) flag[w] = wave_ovf ? 1 : 0;
-        __syncthreads();  // tables and flags are visible
+        if constexpr (!PL::kAdj) {
+          // ---- first windows: the first G + W - 1 columns of each side, in
+          // staging jobs whose windows the wave computes itself (lane t: job
+          // t), so they need no barrier after the table pass; stored before
+          // the one barrier that publishes tables, flags and windows (a row
+          // that then falls back to the gather path leaves them unread).
+          // FP32 single rows go straight into LDS, summed or scaled rows
+          // through registers.
+          constexpr int kFill = (2 * (G + W - 1) + W - 1) / W;
+          const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
+              const_cast<char*>(base + (size_t)r0 * row_bytes), 0, (int)row_bytes, kRsrcWord3);
+          const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(
+              const_cast<char*>(base + (size_t)rm * row_bytes), 0, (int)row_bytes, kRsrcWord3);
+          const const_f32_ptr sc0 = as_const(a.scale + (size_t)r0 * row_tr);
+          const const_f32_ptr sc1 = as_const(a.scale + (size_t)rm * row_tr);
+          const int nc = min(G + W - 1, nO);
+          const StageJob jl = job_at(0, nc, w + min(lane, kFill - 1) * W);
+          int lo_l = 0, wb_l = 0;
+          bool unused_ovf = false;
+          if (jl.ok) col_window(jl.side, jl.o, lo_l, wb_l, unused_ovf);
+#pragma unroll
+          for (int t = 0; t < kFill; ++t) {
+            const StageJob jb = job_at(0, nc, w + t * W);
+            if (jb.ok) {
+              const int lo = __builtin_amdgcn_readlane(lo_l, t);
+              const int wb = __builtin_amdgcn_readlane(wb_l, t);
+              const int k = jb.side == 0 ? kbase + jb.o : kbase + (W - 1) - jb.o;
+              const bool data = k >= xd_lo && k < xd_hi;
+              const int soff = data ? (k - xd_lo) * tb : 0;
+              const int si = data ? k - xd_lo : a.n_xd;
+              const int voff = (!data || lb >= wb) ? 0x7ffffff0 : (lo * kB + lb) >> PL::kHShift;
+              char* const dst = smem + PL::slot(jb.side, jb.o);
+              if (S::kScaled || two) {
+                F4 x = widen(load_row(rs0, voff, soff), S::kScaled ? sc0[si] : 1.0f);
+                if (two) x = widen_add(load_row(rs1, voff, soff), S::kScaled ? sc1[si] : 1.0f, x);
+                *reinterpret_cast<F4*>(dst + lb) = x;
+              } else {
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rs0, (__attribute__((address_space(3))) void*)dst, 16, voff, soff, 0, 0);
+              }
+            }
+          }
+          __builtin_amdgcn_s_waitcnt(0);
+        }
+        __syncthreads();  // tables, flags and (forward) the first windows are visible
         fallback = __any(pl && flag[pl ? lane : 0] != 0);
       }
       if constexpr (PL::kAdj) {
@@ -1195,11 +1300,6 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // the trace end are loaded out of range; columns outside the data
         // are zeros (out of range in the first fill, the row's zero trace
         // afterwards).
-        using V4 = unsigned __attribute__((ext_vector_type(4)));
-        using V2 = unsigned __attribute__((ext_vector_type(2)));
-        using F4 = float __attribute__((ext_vector_type(4)));
-        using JV = std::conditional_t<S::kScaled, V2, V4>;  // one row of a job, per lane
-        const int lb = lane * 16;
         const int lbs = lb < PL::kSlot ? lb : 0x7ff00000;  // lanes past the slot load nothing
         const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<char*>(base + (size_t)r0 * row_bytes), 0, (int)row_bytes, kRsrcWord3);
@@ -1209,102 +1309,10 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // row's zero trace)
         const const_f32_ptr sc0 = as_const(a.scale + (size_t)r0 * row_tr);
         const const_f32_ptr sc1 = as_const(a.scale + (size_t)rm * row_tr);
-        auto load_row = [&](__amdgpu_buffer_rsrc_t r, int voff, int soff) -> JV {
-          if constexpr (S::kScaled)
-            return __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-          else
-            return __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-        };
-        // the two stored pairs of one row as FP32 pairs (sample, difference),
-        // unscaled
-        auto unpack = [&](JV v) -> F4 {
-          if constexpr (ST == PSM_STORE_F32) {
-            return __builtin_bit_cast(F4, v);
-          } else if constexpr (ST == PSM_STORE_F16) {
-            return F4{(float)__builtin_bit_cast(_Float16, (unsigned short)(v.x & 0xffffu)),
-                      (float)__builtin_bit_cast(_Float16, (unsigned short)(v.x >> 16)),
-                      (float)__builtin_bit_cast(_Float16, (unsigned short)(v.y & 0xffffu)),
-                      (float)__builtin_bit_cast(_Float16, (unsigned short)(v.y >> 16))};
-          } else {
-            return F4{__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xffff0000u),
-                      __uint_as_float(v.y << 16), __uint_as_float(v.y & 0xffff0000u)};
-          }
-        };
-        // one row's pairs times its trace's scale s (F16 / BF16; FP32 rows
-        // are not scaled).  fma(x, s, 0) is x s but for the sign of an exact
-        // zero (no matter in a sum), and lets an F16 x be widened and scaled
-        // by one v_fma_mix_f32.
-        auto widen = [&](JV v, float s) -> F4 {
-          if constexpr (S::kScaled) {
-            const F4 x = unpack(v);
-            return F4{__builtin_fmaf(x.x, s, 0.0f), __builtin_fmaf(x.y, s, 0.0f),
-                      __builtin_fmaf(x.z, s, 0.0f), __builtin_fmaf(x.w, s, 0.0f)};
-          }
-          return unpack(v);
-        };
-        // acc + the row's pairs (times s for F16 / BF16)
-        auto widen_add = [&](JV v, float s, F4 acc) -> F4 {
-          const F4 x = unpack(v);
-          if constexpr (S::kScaled)
-            return F4{__builtin_fmaf(x.x, s, acc.x), __builtin_fmaf(x.y, s, acc.y),
-                      __builtin_fmaf(x.z, s, acc.z), __builtin_fmaf(x.w, s, acc.w)};
-          return acc + x;
-        };
-        // A job's source: column o, side of the job index j within the
-        // columns [oa, oa + nc).
-        // (one return statement: a lambda returning this struct on two paths
-        // made the compiler copy the kernel arguments to scratch)
-        auto job_at = [&](int oa, int nc, int j) {
-          return StageJob{oa + (j >> 1), j & 1, 0, j < 2 * nc};
-        };
-        // buffer offsets of a job whose table entry is m (meta of its column)
-        // and its trace's index in the scale table (mw = the column's window
-        // bytes in LDS: lanes past pairs lo .. hi load nothing)
-        auto job_src = [&](const StageJob& jb, int m, int mw, int& voff, int& soff, int& si) {
-          const int k = jb.side == 0 ? kbase + jb.o : kbase + (W - 1) - jb.o;
-          const bool data = k >= xd_lo && k < xd_hi;
-          const int lo = uniform_i((PL::slot(jb.side, jb.o) - m) / kB);
-          soff = data ? (k - xd_lo) * tb : 0;
-          si = data ? k - xd_lo : a.n_xd;
-          const bool zero = !data || lb >= mw;
-          voff = zero ? 0x7ffffff0 : (lo * kB + lb) >> PL::kHShift;
-        };
-        auto job_dst = [&](const StageJob& jb) { return PL::slot(jb.side, jb.o); };
         auto tab = [&](int side, int o) { return meta[side * PL::kCols + o]; };
         auto wtab = [&](int side, int o) {
           return reinterpret_cast<const int*>(smem + PL::kWid)[side * PL::kCols + o];
         };
-        {  // the first G + W - 1 columns (no registers held across the
-           // barrier): FP32 single rows straight into LDS, summed or scaled
-           // rows through registers; complete before the barrier
-          constexpr int kFill = (2 * (G + W - 1) + W - 1) / W;
-          const int nc = min(G + W - 1, nO);
-          int m[kFill], mw[kFill];
-#pragma unroll
-          for (int t = 0; t < kFill; ++t) {
-            const StageJob jb = job_at(0, nc, w + t * W);
-            m[t] = jb.ok ? tab(jb.side, jb.o) : 0;
-            mw[t] = jb.ok ? wtab(jb.side, jb.o) : 0;
-          }
-#pragma unroll
-          for (int t = 0; t < kFill; ++t) {
-            const StageJob jb = job_at(0, nc, w + t * W);
-            if (jb.ok && lb < PL::kSlot) {
-              int voff, soff, si;
-              job_src(jb, m[t], mw[t], voff, soff, si);
-              if (S::kScaled || two) {
-                F4 x = widen(load_row(rs0, voff, soff), S::kScaled ? sc0[si] : 1.0f);
-                if (two) x = widen_add(load_row(rs1, voff, soff), S::kScaled ? sc1[si] : 1.0f, x);
-                *reinterpret_cast<F4*>(smem + job_dst(jb) + lb) = x;
-              } else {
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    rs0, (__attribute__((address_space(3))) void*)(smem + job_dst(jb)), 16, voff,
-                    soff, 0, 0);
-              }
-            }
-          }
-          __builtin_amdgcn_s_waitcnt(0);
-        }
         // Table values of one group, read one group ahead so that their LDS
         // latency overlaps the stores and the barrier: r^2, the two column
         // addresses of each step, and the table entries of the staging jobs
@@ -1501,7 +1509,8 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           }
           Tab t;
           read_tab(0, t);
-          __syncthreads();  // the first windows are visible
+          // (the first windows were published by the barrier after the
+          // table pass)
           if (kmax_wg + 1 >= G)
             run_group(nr_c, T{}, T{}, T{}, 0, t, va, da, sa, vb, db, sb);
           else
