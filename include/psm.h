@@ -178,9 +178,11 @@ int psm_migrate_volume_noalloc(psm_ctx* ctx, const psm_survey* survey,
  * d[i] += w (1 - f) m, d[i+1] += w f m, so <M x, y> = <x, M^T y> up to float
  * rounding.  Crossline padding traces are zero inputs of the forward
  * operator and receive nothing.  Memory per `where`, as psm_migrate_volume.
- * The staged form keeps a row-pair buffer of up to 8 GiB on the context
- * (one plane per output inline and row offset of a pass; psm_set_tuning's
- * rows_per_pass > 0 caps the row offsets per pass).
+ * The staged form keeps a row-pair buffer on the context of up to 8 GiB and
+ * half the free device memory (one plane per output inline and row offset
+ * of a pass; psm_set_tuning's rows_per_pass > 0 caps the row offsets per
+ * pass); without room for one plane it adds each window into both data rows
+ * directly.
  */
 int psm_demigrate_volume(psm_ctx* ctx, const psm_survey* survey,
                          const float* image, const float* vel, float* data,

@@ -632,12 +632,22 @@ int volume_impl(psm_ctx* c, const psm_survey* s, const float* in, const float* v
   const size_t per_d = plan.n_out_il * (size_t)s->n_xl * nt;  // floats per row offset
   int nd = 0;
   if (adjoint && psm::adjoint_staged(a, c->tune) && per_d > 0) {
-    const size_t fit = kAdjRowsBytes / (per_d * sizeof(float));
+    // at most kAdjRowsBytes and half of the free device memory (beyond what
+    // the context holds already)
+    size_t free_b = 0, total_b = 0;
+    size_t budget = kAdjRowsBytes;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+      budget = std::min(budget, (free_b + c->adjrows.cap) / 2);
+    const size_t fit = budget / (per_d * sizeof(float));
     nd = (int)std::min<size_t>(fit, (size_t)extent + 1);
     if (c->tune.rows_per_pass > 0) nd = std::min(nd, c->tune.rows_per_pass);
+    // no buffer: the windows are added into both data rows directly
+    if (nd > 0 && c->adjrows.ensure((size_t)nd * per_d * sizeof(float)) != hipSuccess) {
+      (void)hipGetLastError();
+      nd = 0;
+    }
   }
   if (nd > 0) {
-    PSM_ALLOC(c, c->adjrows, (size_t)nd * per_d * sizeof(float));
     a.adj_rows = c->adjrows.as<float>();
     a.adj_nd = nd;
     // per pass and data row r, the planes io * nd + (d - d0) of the
