@@ -730,7 +730,10 @@ struct LdsPlan {
   // leave room for fewer table entries)
   static constexpr int kMaxColsP = kAdj ? kMaxCols / 2 : kMaxCols;
   static constexpr int kCols = kMaxColsP + 4 * G + W;    // table entries (reads past nO stay inside)
-  static constexpr int kTables = 5 * kCols * 4 + 5 * W * 4 + W * 4;
+  // bytes of a window-table entry: forward (cube bytes of the window's
+  // first pair, LDS bytes of its pairs lo .. hi), adjoint (samples flushed)
+  static constexpr int kWidB = kAdj ? 4 : 8;
+  static constexpr int kTables = 3 * kCols * 4 + 2 * kCols * kWidB + 5 * W * 4 + W * 4;
   // Double ring (2G + W - 1 slots per ring row and side): the next group's
   // windows are stored while the current group is read, one barrier per
   // group.  Single ring (G + W - 1 slots): they are stored after a barrier
@@ -747,10 +750,11 @@ struct LdsPlan {
   static constexpr int kR2 = kMeta + 2 * kCols * 4;      // float r2tab[kCols]
   static constexpr int kPar = kR2 + kCols * 4;           // float par[5][W]
   static constexpr int kFlag = kPar + 5 * W * 4;         // int flag[W]
-  // int wid[2][kCols]: forward, bytes of the column's window (pairs lo ..
-  // hi); adjoint, samples the column flushes
+  // wid[2][kCols]: forward, int2 (lo * kHB: the cube byte offset of the
+  // window's first pair in its trace; bytes of the window's pairs lo .. hi
+  // in LDS); adjoint, int: samples the column flushes
   static constexpr int kWid = kFlag + W * 4;
-  static constexpr int kBytes = kWid + 2 * kCols * 4;
+  static constexpr int kBytes = kWid + 2 * kCols * kWidB;
   // waves per SIMD the LDS allows (three 8-wave workgroups per CU with the
   // double ring), else no register target
   static constexpr int kWavesPerEu = (W == 8 && !kSingle && 3 * (kBytes + 16) <= 160 * 1024) ? 6 : 1;
@@ -1039,13 +1043,22 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         for (int tj = w; (tj >> 1) * kWave < nO + kPast; tj += W) {
           const int side = tj & 1;
           const int o = (tj >> 1) * kWave + lane;
-          int* const wid = reinterpret_cast<int*>(smem + PL::kWid) + side * PL::kCols + o;
+          char* const wid = smem + PL::kWid + (side * PL::kCols + o) * PL::kWidB;
+          using I2 = int __attribute__((ext_vector_type(2)));
           if (o >= nO) {
-            if (o < nO + kPast) *wid = 0;
+            if (o < nO + kPast) {
+              if constexpr (PL::kAdj)
+                *reinterpret_cast<int*>(wid) = 0;
+              else
+                *reinterpret_cast<I2*>(wid) = I2{0, 0};
+            }
           } else {
             int lo, wb;
             col_window(side, o, lo, wb, ovf);
-            *wid = wb;
+            if constexpr (PL::kAdj)
+              *reinterpret_cast<int*>(wid) = wb;
+            else
+              *reinterpret_cast<I2*>(wid) = I2{(lo * kB) >> PL::kHShift, wb};
             meta[side * PL::kCols + o] = PL::slot(side, o) - lo * kB;
           }
         }
@@ -1300,7 +1313,7 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         // the trace end are loaded out of range; columns outside the data
         // are zeros (out of range in the first fill, the row's zero trace
         // afterwards).
-        const int lbs = lb < PL::kSlot ? lb : 0x7ff00000;  // lanes past the slot load nothing
+        const int lbh = lb >> PL::kHShift;  // the lane's cube byte offset in a window
         const __amdgpu_buffer_rsrc_t rs0 = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<char*>(base + (size_t)r0 * row_bytes), 0, (int)row_bytes, kRsrcWord3);
         const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(
@@ -1310,8 +1323,10 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
         const const_f32_ptr sc0 = as_const(a.scale + (size_t)r0 * row_tr);
         const const_f32_ptr sc1 = as_const(a.scale + (size_t)rm * row_tr);
         auto tab = [&](int side, int o) { return meta[side * PL::kCols + o]; };
+        // a column's window entry: (lo * kHB, bytes of pairs lo .. hi in LDS)
         auto wtab = [&](int side, int o) {
-          return reinterpret_cast<const int*>(smem + PL::kWid)[side * PL::kCols + o];
+          using I2 = int __attribute__((ext_vector_type(2)));
+          return *reinterpret_cast<const I2*>(smem + PL::kWid + (side * PL::kCols + o) * PL::kWidB);
         };
         // Table values of one group, read one group ahead so that their LDS
         // latency overlaps the stores and the barrier: r^2, the two column
@@ -1356,8 +1371,9 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           // exists: the index stays inside the table)
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
-            t.mj[q] = tab(js[q], jo[q]);
-            t.mw[q] = wtab(js[q], jo[q]);
+            const auto e = wtab(js[q], jo[q]);
+            t.mj[q] = e.x;
+            t.mw[q] = e.y;
           }
         };
         // Steps dk0 + u, u < n (FULL: all G).  Every wave sweeps the
@@ -1432,17 +1448,16 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
           constexpr int NR = decltype(nr_c)::value;
 #pragma unroll
           for (int q = 0; q < kJobs; ++q) {
-            // a job loads pairs lo .. lo + kWinPairs - 1 of its column (lo * kB
-            // = slot - table entry m); lanes whose pairs all lie past the
+            // a job loads pairs lo .. lo + kWinPairs - 1 of its column (m =
+            // lo * kHB, the window's cube byte offset); lanes whose pairs all lie past the
             // trace end load out of range (no read reaches those slot
             // entries: reads stop at hi <= n_t - 1), the lane holding pair
             // n_t - 1 ends its load there (lo makes n_t - 1 - lo odd)
             const bool data = (unsigned)jkd[q] < (unsigned)a.n_xd;
             dst[q] = jo[q] < jlim[q] ? jsl[q] : PL::kSpare;
-            const int voff = jsl[q] + lbs - m[q];  // lo * kB + lane * 16 (LDS bytes)
             // (lanes past the window's pairs lo .. hi -- hi <= n_t - 1 --
             // load nothing)
-            const int vo = lb < mw[q] ? voff >> PL::kHShift : 0x7ff00000;
+            const int vo = lb < mw[q] ? m[q] + lbh : 0x7ff00000;
             const int so = data ? jso[q] : a.n_xd * tb;
             v[q] = load_row(rs0, vo, so);
             if constexpr (NR == 2) v[kJobs + q] = load_row(rs1, vo, so);
@@ -1502,8 +1517,9 @@ __device__ __forceinline__ void staged_tile(const MigArgs a, const Task tk, int 
             int m0[kJobs], w0[kJobs];
 #pragma unroll
             for (int q = 0; q < kJobs; ++q) {
-              m0[q] = tab(js[q], jo[q]);
-              w0[q] = wtab(js[q], jo[q]);
+              const auto e = wtab(js[q], jo[q]);
+              m0[q] = e.x;
+              w0[q] = e.y;
             }
             issue(nr_c, m0, w0, va, da, sa);
           }
