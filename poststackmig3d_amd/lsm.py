@@ -15,11 +15,15 @@ inner products and residual norms are accumulated in float64 on the GPU
 gamma, iteration, residual history) is written every `checkpoint_every`
 iterations; `resume=path` continues from such a file.  Checkpoints are
 written behind the solver: the state is copied into page-locked host buffers
-in stream order and a background thread writes the file, so only the last
-checkpoint of a call is waited for.
+in stream order and a background thread writes the file (to path.tmp, then
+renamed over path, so the file is always a whole checkpoint).  The last
+checkpoint of a call may still be in flight when cgls returns; it is waited
+for by the next checkpoint written to the same path, by cgls(resume=path),
+by wait_checkpoints() and at interpreter exit.
 """
 from __future__ import annotations
 
+import atexit
 import math
 import os
 import threading
@@ -48,6 +52,33 @@ def _save(path, state):
 _pinned: dict = {}
 _pinned_lock = threading.Lock()
 
+# writers whose last checkpoint was still being written when their cgls call
+# returned, by path
+_pending: dict = {}
+_pending_lock = threading.Lock()
+
+
+def _take_pending(path):
+    with _pending_lock:
+        return _pending.pop(path, None)
+
+
+def wait_checkpoints(path: str | None = None) -> None:
+    """Blocks until the checkpoint write still in flight for `path` (every
+    path if None) is on disk; re-raises its error, if any."""
+    if path is not None:
+        writers = [_take_pending(path)]
+    else:
+        with _pending_lock:
+            writers = list(_pending.values())
+            _pending.clear()
+    for w in writers:
+        if w is not None:
+            w.close()
+
+
+atexit.register(wait_checkpoints)
+
 
 class _CheckpointWriter:
     """Writes checkpoints without stalling the solver.  submit() copies the
@@ -56,13 +87,20 @@ class _CheckpointWriter:
     cannot overtake the copy -- and a background thread waits for the copy
     and writes the file (to path.tmp, then renamed over path).  One write is
     in flight at a time; wait() blocks until it is on disk and re-raises its
-    error, if any."""
+    error, if any.  A write left in flight by an earlier call on the same
+    path is waited for before this writer's first write."""
 
     def __init__(self, path):
         self.path = path
         self.thread = None
         self.error = None
         self.bufs = {}
+        self.prev = _take_pending(path)
+
+    def _finish_prev(self):
+        if self.prev is not None:
+            prev, self.prev = self.prev, None
+            prev.close()
 
     def _buffer(self, name, t):
         import torch
@@ -80,6 +118,7 @@ class _CheckpointWriter:
     def submit(self, state):
         import torch
         self.wait()
+        self._finish_prev()
         host = {}
         for k, v in state.items():
             if torch.is_tensor(v) and v.is_cuda:
@@ -112,13 +151,31 @@ class _CheckpointWriter:
             raise e
 
     def close(self):
+        """Waits for the write in flight (and an earlier call's on the same
+        path) and returns the page-locked buffers to the pool."""
         try:
-            self.wait()
+            self._finish_prev()
         finally:
-            with _pinned_lock:
-                for b in self.bufs.values():
-                    _pinned.setdefault((tuple(b.shape), b.dtype), []).append(b)
-            self.bufs = {}
+            try:
+                self.wait()
+            finally:
+                with _pinned_lock:
+                    for b in self.bufs.values():
+                        _pinned.setdefault((tuple(b.shape), b.dtype), []).append(b)
+                self.bufs = {}
+
+    def detach(self):
+        """End of a call: a write still in flight is left to finish behind
+        the caller (registered by path, its buffers held until it is waited
+        for); otherwise the buffers go back to the pool now."""
+        if self.thread is None:
+            self.close()
+            return
+        with _pending_lock:
+            earlier = _pending.pop(self.path, None)
+            _pending[self.path] = self
+        if earlier is not None:   # (a path shared by concurrent calls)
+            earlier.close()
 
 
 def cgls(data, vel, params: SurveyParams, iterations: int = 10, model0=None,
@@ -127,7 +184,9 @@ def cgls(data, vel, params: SurveyParams, iterations: int = 10, model0=None,
          stop_after: int | None = None) -> CGResult:
     """CGLS least-squares migration.  data, vel: device tensors
     [n_il][n_xl][n_t].  stop_after: return after that many iterations of this
-    call (the state is checkpointed first), for staged runs."""
+    call (the state is checkpointed first), for staged runs.  The last
+    checkpoint may still be being written when this returns (see
+    wait_checkpoints)."""
     import torch
 
     ctx = ctx or _lib.context()
@@ -136,6 +195,7 @@ def cgls(data, vel, params: SurveyParams, iterations: int = 10, model0=None,
     dot = lambda a, b: dot_f64(a, b, ctx=ctx)                      # noqa: E731
 
     if resume is not None:
+        wait_checkpoints(resume)
         st = torch.load(resume, map_location=data.device, weights_only=True)
         m, r, p = st["m"], st["r"], st["p"]
         gamma = float(st["gamma"])
@@ -162,9 +222,15 @@ def cgls(data, vel, params: SurveyParams, iterations: int = 10, model0=None,
     try:
         k = _iterate(F, Ft, dot, m, r, p, gamma, k0, rn, gn, iterations, writer,
                      checkpoint_every, stop_after)
-    finally:
+    except BaseException:
         if writer is not None:
-            writer.close()   # the last checkpoint is on disk before returning
+            try:
+                writer.close()
+            except Exception:
+                pass   # (the solver's error is the one raised)
+        raise
+    if writer is not None:
+        writer.detach()   # the last write finishes behind the caller
     return CGResult(model=m, residual_norms=rn, gradient_norms=gn, iterations=k)
 
 

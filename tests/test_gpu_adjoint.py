@@ -154,3 +154,34 @@ def test_cgls_decreases_residual_and_resumes(tmp_path):
     assert resumed.iterations == 6
     assert rel_l2(resumed.model.cpu().numpy(), full.model.cpu().numpy()) <= 1e-5
     np.testing.assert_allclose(resumed.residual_norms, rn, rtol=1e-5)
+
+
+def test_cgls_last_checkpoint_written_behind_the_caller(tmp_path):
+    """A solve returns with its last checkpoint possibly still being written;
+    a second solve on the same path waits for it before its own first write,
+    wait_checkpoints() leaves the last solve's final state on disk, and
+    resume reads only whole checkpoints."""
+    torch = _torch()
+    from poststackmig3d_amd.lsm import cgls, wait_checkpoints
+    n = (41, 37, 201)
+    vel = torch.full(n, 2000.0, device="cuda")
+    prm = P.SurveyParams.for_cube(n, angle_aperture=30.0)
+    m_true = torch.from_numpy(synth.spikes(n, density=0.01, seed=7)).cuda()
+    d = P.demigrate_volume(m_true, vel, prm)
+    ck = str(tmp_path / "cg.pt")
+    first = cgls(d, vel, prm, iterations=4, checkpoint=ck, checkpoint_every=2)
+    second = cgls(d, vel, prm, iterations=5, checkpoint=ck, checkpoint_every=2)
+    wait_checkpoints(ck)
+    st = torch.load(ck, map_location="cpu", weights_only=True)
+    assert int(st["iteration"]) == 5
+    assert rel_l2(st["m"].numpy(), second.model.cpu().numpy()) == 0.0
+    np.testing.assert_allclose(st["residual_norms"], second.residual_norms, rtol=0)
+    # (the adjoint's flush adds with atomics: runs agree to rounding)
+    np.testing.assert_allclose(first.residual_norms, second.residual_norms[:5], rtol=1e-5)
+    # resume from the (possibly still in flight) checkpoint of a stopped solve
+    part = cgls(d, vel, prm, iterations=5, checkpoint=ck, stop_after=2)
+    assert part.iterations == 2
+    resumed = cgls(d, vel, prm, iterations=5, resume=ck)
+    assert resumed.iterations == 5
+    np.testing.assert_allclose(resumed.residual_norms, second.residual_norms, rtol=1e-5)
+    assert not (tmp_path / "cg.pt.tmp").exists()
