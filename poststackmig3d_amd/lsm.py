@@ -24,13 +24,14 @@ by wait_checkpoints() and at interpreter exit.
 from __future__ import annotations
 
 import atexit
+import dataclasses
 import math
 import os
 import threading
 from dataclasses import dataclass, field
 
 from . import _lib
-from .kernel import SurveyParams, demigrate_volume, dot_f64, migrate_volume
+from .kernel import SurveyParams, demigrate_volume, dot_f64, migrate_volume, survey_plan
 
 
 @dataclass
@@ -190,6 +191,11 @@ def cgls(data, vel, params: SurveyParams, iterations: int = 10, model0=None,
     import torch
 
     ctx = ctx or _lib.context()
+    if not params.max_velocity > 0:
+        # every application shares vel: probe its maximum once (as each call
+        # would, main.cpp:22-64), so the calls skip their synchronous probes
+        # and queue behind one another
+        params = dataclasses.replace(params, max_velocity=survey_plan(params, vel).max_velocity)
     F = lambda m: demigrate_volume(m, vel, params, ctx=ctx)       # noqa: E731
     Ft = lambda r: migrate_volume(r, vel, params, ctx=ctx)         # noqa: E731
     dot = lambda a, b: dot_f64(a, b, ctx=ctx)                      # noqa: E731
