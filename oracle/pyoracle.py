@@ -169,9 +169,13 @@ def _vol(data, dt, dx, dy, pad_il, pad_xl, angle, amp):
 
 
 def out_shape(data_shape, pad_il, pad_xl, il_labels=None):
+    """Output inlines and crosslines as main.cpp:140-171 labels them (a single
+    inline or crossline label gives no label step, hence no padding on that
+    axis; the kernel's buffer is padded all the same, main.cpp:296-307)."""
     n_il, n_xl, n_t = data_shape
     ill = np.arange(1, n_il + 1, dtype=np.int32) if il_labels is None else il_labels
-    return output_labels(ill, pad_il).size, n_xl + 2 * pad_xl, n_t
+    xll = np.arange(1, n_xl + 1, dtype=np.int32)
+    return output_labels(ill, pad_il).size, output_labels(xll, pad_xl).size, n_t
 
 
 # ---------------------------------------------------------------- volume
@@ -205,7 +209,7 @@ def ref_migrate_volume(data, vel, dt, dx, dy, pad_il=0, pad_xl=0, angle=30.0, am
     ill = np.arange(1, n_il + 1, dtype=np.int32)
     oll = output_labels(ill, pad_il)
     c_of = np.array([map_output_inline(ill, l) for l in oll], np.int32)
-    shp = (oll.size, n_xl + 2 * pad_xl, n_t)
+    shp = out_shape(data.shape, pad_il, pad_xl)
     out = np.zeros(shp, np.float32)
     rws = None if rows is None else np.ascontiguousarray(rows, np.int32)
     ref().psmr_migrate_volume(_f(data), _f(vel), n_il, n_xl, n_t, dt, dx, dy, pad_il, pad_xl,
