@@ -121,21 +121,28 @@ class _CheckpointWriter:
         self.wait()
         self._finish_prev()
         host = {}
+        on_device = False
         for k, v in state.items():
             if torch.is_tensor(v) and v.is_cuda:
                 b = self._buffer(k, v)
                 b.copy_(v, non_blocking=True)
                 host[k] = b
+                on_device = True
+            elif torch.is_tensor(v):
+                host[k] = v.clone()   # (host tensors: a copy the solver cannot overtake)
             elif isinstance(v, list):
                 host[k] = list(v)
             else:
                 host[k] = v
-        done = torch.cuda.Event()
-        done.record()
+        done = None
+        if on_device:
+            done = torch.cuda.Event()
+            done.record()
 
         def write():
             try:
-                done.synchronize()
+                if done is not None:
+                    done.synchronize()
                 _save(self.path, host)
             except BaseException as e:  # re-raised by wait()
                 self.error = e
