@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <iostream>
 #include <memory>
 #include <string>
@@ -210,13 +211,32 @@ void write_segy(const char* path, const psmh_segy* in, const int32_t* oil, size_
   if (std::fwrite(head.data(), 1, head.size(), f) != head.size())
     throw std::runtime_error("write failed: " + std::string(path));
   const size_t tr_len = psmh::kTraceHeader + 4 * n_t;
-  std::vector<unsigned char> line(n_oxl * tr_len);
-  for (size_t io = 0; io < n_oil; ++io) {
-    const int32_t il_label = oil[io];
-    const bool real_il = std::binary_search(in->il.begin(), in->il.end(), il_label);
+  // Output inlines in chunks of about 16 MB: a chunk is encoded (OpenMP over
+  // its traces) into one of two buffers while a second thread writes the
+  // previous chunk from the other; the file is written in inline order.
+  const size_t line_bytes = n_oxl * tr_len;
+  const size_t chunk_lines = std::max<size_t>(1, ((size_t)16 << 20) / std::max<size_t>(line_bytes, 1));
+  std::vector<unsigned char> bufs[2];
+  for (auto& b : bufs) b.resize(std::min(chunk_lines, n_oil) * line_bytes);
+  std::thread writer;
+  bool write_ok = true;
+  auto join_writer = [&] {
+    if (writer.joinable()) writer.join();
+  };
+  struct Joiner {  // (joins on every exit, so no thread outlives the buffers)
+    std::function<void()> fn;
+    ~Joiner() { fn(); }
+  } joiner{join_writer};
+  for (size_t c0 = 0, slot = 0; c0 < n_oil; c0 += chunk_lines, slot ^= 1) {
+    const size_t nl = std::min(chunk_lines, n_oil - c0);
+    unsigned char* const chunk = bufs[slot].data();
 #pragma omp parallel for schedule(static)
-    for (long long jo = 0; jo < (long long)n_oxl; ++jo) {
-      unsigned char* t = line.data() + (size_t)jo * tr_len;
+    for (long long q = 0; q < (long long)(nl * n_oxl); ++q) {
+      const size_t io = c0 + (size_t)q / n_oxl;
+      const size_t jo = (size_t)q % n_oxl;
+      const int32_t il_label = oil[io];
+      const bool real_il = std::binary_search(in->il.begin(), in->il.end(), il_label);
+      unsigned char* t = chunk + (size_t)q * tr_len;
       const int64_t k = real_il ? in->find(il_label, oxl[jo]) : -1;
       if (k >= 0) {
         std::memcpy(t, in->trace((size_t)k), psmh::kTraceHeader);
@@ -239,9 +259,14 @@ void write_segy(const char* path, const psmh_segy* in, const int32_t* oil, size_
         std::memcpy(d + 4 * i, &w, 4);
       }
     }
-    if (std::fwrite(line.data(), 1, line.size(), f) != line.size())
-      throw std::runtime_error("write failed: " + std::string(path));
+    join_writer();  // (the previous chunk is written; its buffer is the next one's)
+    if (!write_ok) throw std::runtime_error("write failed: " + std::string(path));
+    writer = std::thread([&write_ok, f, chunk, n = nl * line_bytes] {
+      if (std::fwrite(chunk, 1, n, f) != n) write_ok = false;
+    });
   }
+  join_writer();
+  if (!write_ok) throw std::runtime_error("write failed: " + std::string(path));
   if (std::fflush(f) != 0) throw std::runtime_error("write failed: " + std::string(path));
 }
 

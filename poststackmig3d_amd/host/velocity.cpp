@@ -43,14 +43,16 @@ bool is_segy(const std::string& p) {
 // SegyVelocityProvider over the velocity file's own grid (:37-261).
 struct SegyVel {
   psmh_segy* f = nullptr;
-  std::vector<float> cube;  // [n_il][n_xl][n_t] on the velocity grid
+  // [n_il][n_xl][n_t] on the velocity grid (every element written by the
+  // decode, so left uninitialised until then)
+  std::unique_ptr<float[]> cube;
   size_t n_t = 0;
   double dt = 0.0;
   std::vector<double> times;
   ~SegyVel() { psmh_segy_close(f); }
 
   // readTraceByCoords: a missing trace reads as zeros (segy_utils.cpp:223-241)
-  const float* trace(size_t a, size_t b) const { return cube.data() + (a * f->xl.size() + b) * n_t; }
+  const float* trace(size_t a, size_t b) const { return cube.get() + (a * f->xl.size() + b) * n_t; }
   std::vector<float> zeros;
 
   // linear interpolation in time (:124-138, :183-196), double arithmetic
@@ -161,8 +163,18 @@ void velocity_cube(psm_ctx* ctx, const char* param_c, const int32_t* il, size_t 
     sv.dt = sv.f->dt_us * 1e-6;
     sv.times.resize(sv.n_t);
     for (size_t i = 0; i < sv.n_t; ++i) sv.times[i] = (double)i * sv.dt;
-    sv.cube.resize(sv.f->il.size() * sv.f->xl.size() * sv.n_t);
-    rc = psmh_segy_read_cube(sv.f, sv.cube.data());
+    // the velocity file on the data's own grid (same labels, dt and sample
+    // count): every trace is an exact copy (getVelocityIline's exact path,
+    // a missing trace reading as zeros), so decode straight into vel
+    if (sv.f->il.size() == n_il && sv.f->xl.size() == n_xl && sv.n_t == n_t &&
+        std::abs(dt - sv.dt) < 1e-6 && std::equal(il, il + n_il, sv.f->il.begin()) &&
+        std::equal(xl, xl + n_xl, sv.f->xl.begin())) {
+      rc = psmh_segy_read_cube(sv.f, vel);
+      if (rc != PSM_OK) throw std::runtime_error(psmh_last_error());
+      return;
+    }
+    sv.cube.reset(new float[sv.f->il.size() * sv.f->xl.size() * sv.n_t]);
+    rc = psmh_segy_read_cube(sv.f, sv.cube.get());
     if (rc != PSM_OK) throw std::runtime_error(psmh_last_error());
 #pragma omp parallel for schedule(dynamic, 1)
     for (long long a = 0; a < (long long)n_il; ++a)
