@@ -13,6 +13,7 @@
 #include <iostream>
 #include <memory>
 #include <string>
+#include <thread>
 
 #include "host_internal.h"
 
@@ -69,6 +70,18 @@ void run(const char* config_path, int device) {
             << bar << std::endl;
 
   std::cout << "\n[1/5] Reading the input cube..." << std::endl;
+  // the device context (HIP start-up) is created while the input file's
+  // headers are scanned; it is waited for before the first page-locked
+  // buffer is allocated
+  Ctx ctx;
+  int ctx_rc = PSM_OK;
+  std::thread ctx_thread([&] { ctx_rc = psm_create(&ctx.c, device); });
+  struct JoinCtx {
+    std::thread& t;
+    ~JoinCtx() {
+      if (t.joinable()) t.join();
+    }
+  } join_ctx{ctx_thread};
   Segy in;
   if (psmh_segy_open(cfg.input_data, &in.s) != PSM_OK) throw std::runtime_error(psmh_last_error());
   size_t n_traces = 0, n_t = 0, n_il = 0, n_xl = 0;
@@ -80,12 +93,11 @@ void run(const char* config_path, int device) {
             << " crosslines, data format " << fmt << (fmt == 1 ? " (IBM Float)" : " (IEEE Float)")
             << "\nInput geometry: IL=" << n_il << ", XL=" << n_xl << ", T=" << n_t << ", dt=" << dt << "s"
             << std::endl;
+  ctx_thread.join();
+  check(ctx_rc, "cannot create a context on GPU " + std::to_string(device));
   HostBuf cube_buf, vel_buf, out_buf;
   float* cube = cube_buf.alloc(n_il * n_xl * n_t);
   if (psmh_segy_read_cube(in.s, cube) != PSM_OK) throw std::runtime_error(psmh_last_error());
-
-  Ctx ctx;
-  check(psm_create(&ctx.c, device), "cannot create a context on GPU " + std::to_string(device));
 
   std::cout << "\n[2/5] Building the velocity cube..." << std::endl;
   float* vel = vel_buf.alloc(n_il * n_xl * n_t);
